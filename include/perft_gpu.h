@@ -1,0 +1,136 @@
+/*
+ * perft_gpu.h -- C ABI of the MI355X (gfx950) perft engine.
+ *
+ * Plain pointers and sizes only.  Every entry point returns PG_OK (0) or a
+ * negative PG_E* code and never exits the process.  Output buffers are
+ * caller-owned host memory; the library owns all device memory.
+ *
+ * The reference (ankan-ban/perft_gpu) has no plugin API: its boundary is the
+ * set of C++ functions main() and launcher.h call.  Each entry below names
+ * the reference interface it replaces.
+ */
+#ifndef PERFT_GPU_H
+#define PERFT_GPU_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Byte-identical to the reference's HexaBitBoardPosition (chess.h:208-230):
+ * six bitboards, game state packed in the rank-1/rank-8 bytes of `pawns`
+ * (bits 0-1 white castle, 2-3 black castle, 4-7 en-passant file + 1,
+ * 56-62 half-move counter, 63 side to move). */
+typedef struct pg_pos48 {
+    uint64_t whitePieces, pawns, knights, bishopQueens, rookQueens, kings;
+} pg_pos48;
+
+/* HashKey128b (chess.h:411-452): lowPart, highPart */
+typedef struct pg_key128 {
+    uint64_t lo, hi;
+} pg_key128;
+
+/* Moves are the reference's 16-bit CMove (chess.h:234-293):
+ * from | to << 6 | flags << 12. */
+typedef uint16_t pg_move;
+
+enum {
+    PG_OK = 0,
+    PG_EINVAL = -1,   /* bad argument */
+    PG_ENOMEM = -2,   /* device memory budget exhausted */
+    PG_EHIP = -3,     /* HIP runtime error */
+    PG_ENODEV = -4,   /* no usable GPU */
+    PG_ETABLE = -5,   /* lookup-table construction failed */
+    PG_EOVERFLOW = -6 /* a result does not fit the requested type */
+};
+
+/* Slider-attack variants (switches.h:102-125 selects one at compile time in
+ * the reference; here each is a kernel instantiation). */
+enum {
+    PG_SLIDER_AUTO = -1,         /* the context's default (autotuned) */
+    PG_SLIDER_KOGGE_STONE = 0,   /* USE_SLIDING_LUT 0 */
+    PG_SLIDER_MAGIC_PLAIN = 1,   /* USE_FANCY_MAGICS 0 */
+    PG_SLIDER_MAGIC_FANCY = 2,   /* USE_FANCY_MAGICS 1 (reference default) */
+    PG_SLIDER_MAGIC_BYTE = 3     /* USE_BYTE_LOOKUP_FANCY 1, tables in LDS */
+};
+
+typedef struct pg_config {
+    uint64_t arena_bytes;  /* device tree arena; 0 = default (48 GiB, capped by free HBM) */
+    int slider;            /* default slider variant, PG_SLIDER_AUTO = Kogge-Stone */
+    int block_size;        /* work-group size for the BFS kernels, 0 = 256 */
+    int blocks_per_cu;     /* resident work-groups per CU for grid-stride kernels, 0 = 8 */
+    int timing;            /* 1: time each leaf-kernel launch with HIP events */
+} pg_config;
+
+typedef struct pg_stats {
+    double total_ms;          /* wall time of the last perft call */
+    double leaf_kernel_ms;    /* summed duration of leaf-kernel launches (timing=1) */
+    uint64_t leaf_launches;   /* leaf-kernel launches in the last call */
+    uint64_t leaf_items;      /* (parent, move) items the leaf kernels consumed */
+    uint64_t leaf_parents;    /* parent boards the leaf kernels read */
+    uint64_t leaves;          /* the perft result */
+    uint64_t splits;          /* level splits forced by the arena budget */
+    uint64_t peak_arena;      /* peak arena bytes in use */
+    uint64_t nodes[16];       /* boards materialized per ply below the root */
+} pg_stats;
+
+/* ------------------------------------------------------------ host side */
+
+/* Utils::readFENString + Utils::board088ToHexBB (util.cpp:189, :92).
+ * Clocks are not read, as in the reference. */
+int pg_fen_to_pos(const char *fen, pg_pos48 *out);
+/* Inverse, for display and fixtures (Utils::dispBoard has no FEN writer). */
+int pg_pos_to_fen(const pg_pos48 *pos, char *buf, int buflen);
+/* MoveGeneratorBitboard::computeZobristKey128b (MoveGeneratorBitboard.h:3914)
+ * with the reference's key sets (randoms.cpp via :985-986). */
+int pg_zobrist128(const pg_pos48 *pos, pg_key128 *out);
+/* MoveGeneratorBitboard::generateMoves on the host (:2471); up to 218 moves. */
+int pg_host_generate_moves(const pg_pos48 *pos, pg_move *out, uint32_t *n);
+/* MoveGeneratorBitboard::makeMove<chance, updateHash> (:3575); key may be NULL. */
+int pg_host_make_move(pg_pos48 *pos, pg_move move, pg_key128 *key);
+/* Slider lookups of one variant on the host tables (the device copies are
+ * probed by pg_device_slider_attacks). */
+int pg_host_slider_attacks(int slider, int rook, const uint8_t *squares, const uint64_t *occupancy,
+                           uint32_t n, uint64_t *out);
+/* Seeded workload generator (no reference counterpart): start position plus
+ * a uniform-random number of plies in [min_ply, max_ply], each ply a uniform
+ * pick among the legal moves sorted by (from, to, promotion); games that end
+ * early or final positions without a legal move are redrawn. */
+int pg_random_positions(uint64_t seed, uint32_t n, uint32_t min_ply, uint32_t max_ply,
+                        pg_pos48 *out, uint32_t *plies);
+
+/* ---------------------------------------------------------- device side */
+
+typedef struct pg_ctx pg_ctx;
+
+/* initGPU (launcher.h:1506) + MoveGeneratorBitboard::init (:982):
+ * selects the device, builds and uploads tables, reserves the arena. */
+int pg_init(int device, const pg_config *cfg, pg_ctx **ctx);
+void pg_shutdown(pg_ctx *ctx);
+int pg_get_config(pg_ctx *ctx, pg_config *out);
+int pg_set_slider(pg_ctx *ctx, int slider);
+
+/* perftLauncher, no-TT branch (launcher.h:1455-1503) driving
+ * perft_bb_driver_gpu / perft_bb_gpu_simple (perft_bb.h:1027-1220):
+ * breadth-first perft of one position; returns the leaf count. */
+int pg_perft(pg_ctx *ctx, const pg_pos48 *pos, uint32_t depth, int slider, uint64_t *out);
+int pg_last_stats(pg_ctx *ctx, pg_stats *out);
+/* Text of the last error on the calling thread (empty if none). */
+const char *pg_last_error(void);
+
+/* Device move-generator probes (one thread per board); the reference runs
+ * the same functions inside its kernels (countMoves :3010, generateMoves
+ * :2471, makeMove :3575, slider lookups :731-838).  moves: n*256 slots. */
+int pg_device_count_moves(pg_ctx *ctx, const pg_pos48 *pos, uint32_t n, int slider, uint32_t *out);
+int pg_device_generate_moves(pg_ctx *ctx, const pg_pos48 *pos, uint32_t n, int slider,
+                             pg_move *moves, uint32_t *counts);
+int pg_device_make_move(pg_ctx *ctx, const pg_pos48 *pos, const pg_move *moves, uint32_t n,
+                        pg_pos48 *out, pg_key128 *keys /* in: parent keys, out: child keys; may be NULL */);
+int pg_device_slider_attacks(pg_ctx *ctx, int slider, int rook, const uint8_t *squares,
+                             const uint64_t *occupancy, uint32_t n, uint64_t *out);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

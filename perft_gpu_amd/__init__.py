@@ -1,0 +1,308 @@
+"""perft_gpu_amd -- chess perft on AMD Instinct MI355X (gfx950).
+
+Python face of the C ABI in include/perft_gpu.h (libperft_gpu_amd.so, built
+in-tree).  The functions mirror the reference's host entry points:
+
+  read_fen_string        Utils::readFENString + board088ToHexBB (util.cpp:189, :92)
+  init_gpu               initGPU + MoveGeneratorBitboard::init (launcher.h:1506,
+                         MoveGeneratorBitboard.h:982)
+  perft_launcher         perftLauncher, no-TT branch (launcher.h:1455-1503)
+  zobrist128             computeZobristKey128b (MoveGeneratorBitboard.h:3914)
+
+Perft itself only runs on the GPU: every device entry point raises if the
+library or a GPU is missing -- there is no CPU fallback.
+"""
+import ctypes
+import os
+import time
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libperft_gpu_amd.so")
+
+START_FEN = "rnbqkbnr/pppppppp/8/8/8/8/PPPPPPPP/RNBQKBNR w KQkq - 0 1"
+
+SLIDER_AUTO = -1
+SLIDER_KOGGE_STONE = 0
+SLIDER_MAGIC_PLAIN = 1
+SLIDER_MAGIC_FANCY = 2
+SLIDER_MAGIC_BYTE = 3
+SLIDER_NAMES = {0: "kogge_stone", 1: "magic_plain", 2: "magic_fancy", 3: "magic_byte_lds"}
+
+PG_OK, PG_EINVAL, PG_ENOMEM, PG_EHIP, PG_ENODEV, PG_ETABLE, PG_EOVERFLOW = 0, -1, -2, -3, -4, -5, -6
+_ERR_NAMES = {-1: "EINVAL", -2: "ENOMEM", -3: "EHIP", -4: "ENODEV", -5: "ETABLE", -6: "EOVERFLOW"}
+
+u64 = ctypes.c_uint64
+
+
+class Pos48(ctypes.Structure):
+    """HexaBitBoardPosition (chess.h:208-230), 48 bytes."""
+    _fields_ = [(n, u64) for n in ("whitePieces", "pawns", "knights", "bishopQueens", "rookQueens", "kings")]
+
+    def words(self):
+        return tuple(getattr(self, n) for n, _ in self._fields_)
+
+    def side(self):
+        return int(self.pawns >> 63)
+
+
+class Key128(ctypes.Structure):
+    _fields_ = [("lo", u64), ("hi", u64)]
+
+
+class Config(ctypes.Structure):
+    _fields_ = [("arena_bytes", u64), ("slider", ctypes.c_int), ("block_size", ctypes.c_int),
+                ("blocks_per_cu", ctypes.c_int), ("timing", ctypes.c_int)]
+
+
+class Stats(ctypes.Structure):
+    _fields_ = [("total_ms", ctypes.c_double), ("leaf_kernel_ms", ctypes.c_double),
+                ("leaf_launches", u64), ("leaf_items", u64), ("leaf_parents", u64), ("leaves", u64),
+                ("splits", u64), ("peak_arena", u64), ("nodes", u64 * 16)]
+
+    def as_dict(self):
+        d = {n: getattr(self, n) for n, _ in self._fields_ if n != "nodes"}
+        d["nodes"] = list(self.nodes)
+        return d
+
+
+class PerftError(RuntimeError):
+    def __init__(self, code, what):
+        msg = "%s failed: %s" % (what, _ERR_NAMES.get(code, code))
+        detail = last_error()
+        if detail:
+            msg += " (%s)" % detail
+        super().__init__(msg)
+        self.code = code
+
+
+_lib = None
+
+
+def lib():
+    """Load the in-tree library; raises if it has not been built."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise RuntimeError("libperft_gpu_amd.so is not built (run __graft_entry__.build())")
+    L = ctypes.CDLL(LIB_PATH)
+    P = ctypes.POINTER
+    sig = {
+        "pg_fen_to_pos": (ctypes.c_int, [ctypes.c_char_p, P(Pos48)]),
+        "pg_pos_to_fen": (ctypes.c_int, [P(Pos48), ctypes.c_char_p, ctypes.c_int]),
+        "pg_zobrist128": (ctypes.c_int, [P(Pos48), P(Key128)]),
+        "pg_host_generate_moves": (ctypes.c_int, [P(Pos48), P(ctypes.c_uint16), P(ctypes.c_uint32)]),
+        "pg_host_make_move": (ctypes.c_int, [P(Pos48), ctypes.c_uint16, P(Key128)]),
+        "pg_host_slider_attacks": (ctypes.c_int, [ctypes.c_int, ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p,
+                                                  ctypes.c_uint32, ctypes.c_void_p]),
+        "pg_random_positions": (ctypes.c_int, [u64, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint32,
+                                               ctypes.c_void_p, ctypes.c_void_p]),
+        "pg_init": (ctypes.c_int, [ctypes.c_int, P(Config), P(ctypes.c_void_p)]),
+        "pg_shutdown": (None, [ctypes.c_void_p]),
+        "pg_get_config": (ctypes.c_int, [ctypes.c_void_p, P(Config)]),
+        "pg_set_slider": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int]),
+        "pg_perft": (ctypes.c_int, [ctypes.c_void_p, P(Pos48), ctypes.c_uint32, ctypes.c_int, P(u64)]),
+        "pg_last_stats": (ctypes.c_int, [ctypes.c_void_p, P(Stats)]),
+        "pg_last_error": (ctypes.c_char_p, []),
+        "pg_device_count_moves": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_uint32, ctypes.c_int,
+                                                 ctypes.c_void_p]),
+        "pg_device_generate_moves": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_uint32, ctypes.c_int,
+                                                    ctypes.c_void_p, ctypes.c_void_p]),
+        "pg_device_make_move": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_uint32,
+                                               ctypes.c_void_p, ctypes.c_void_p]),
+        "pg_device_slider_attacks": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.c_void_p,
+                                                    ctypes.c_void_p, ctypes.c_uint32, ctypes.c_void_p]),
+    }
+    for name, (res, args) in sig.items():
+        f = getattr(L, name)
+        f.restype = res
+        f.argtypes = args
+    _lib = L
+    return L
+
+
+def last_error():
+    if _lib is None:
+        return ""
+    s = _lib.pg_last_error()
+    return s.decode() if s else ""
+
+
+def _check(rc, what):
+    if rc != PG_OK:
+        raise PerftError(rc, what)
+
+
+def _ptr(a):
+    return a.ctypes.data_as(ctypes.c_void_p)
+
+
+# ------------------------------------------------------------------ host side
+def read_fen_string(fen):
+    """FEN -> Pos48 (Utils::readFENString + board088ToHexBB)."""
+    p = Pos48()
+    _check(lib().pg_fen_to_pos(fen.encode(), ctypes.byref(p)), "pg_fen_to_pos")
+    return p
+
+
+def pos_to_fen(pos):
+    buf = ctypes.create_string_buffer(128)
+    _check(lib().pg_pos_to_fen(ctypes.byref(pos), buf, 128), "pg_pos_to_fen")
+    return buf.value.decode()
+
+
+def zobrist128(pos):
+    k = Key128()
+    _check(lib().pg_zobrist128(ctypes.byref(pos), ctypes.byref(k)), "pg_zobrist128")
+    return int(k.lo), int(k.hi)
+
+
+def generate_moves_host(pos):
+    buf = (ctypes.c_uint16 * 256)()
+    n = ctypes.c_uint32()
+    _check(lib().pg_host_generate_moves(ctypes.byref(pos), buf, ctypes.byref(n)), "pg_host_generate_moves")
+    return list(buf[:n.value])
+
+
+def make_move_host(pos, move, key=None):
+    """Returns (child, child_key or None)."""
+    child = Pos48(*pos.words())
+    k = None
+    if key is not None:
+        k = Key128(key[0], key[1])
+    _check(lib().pg_host_make_move(ctypes.byref(child), move, ctypes.byref(k) if k else None), "pg_host_make_move")
+    return child, ((int(k.lo), int(k.hi)) if k else None)
+
+
+def slider_attacks_host(slider, rook, squares, occupancy):
+    sq = np.ascontiguousarray(squares, dtype=np.uint8)
+    occ = np.ascontiguousarray(occupancy, dtype=np.uint64)
+    out = np.zeros(len(sq), dtype=np.uint64)
+    _check(lib().pg_host_slider_attacks(slider, int(rook), _ptr(sq), _ptr(occ), len(sq), _ptr(out)),
+           "pg_host_slider_attacks")
+    return out
+
+
+def random_positions(seed, n, min_ply, max_ply):
+    """Seeded workload generator; returns (positions as (n,6) uint64 array, plies)."""
+    arr = np.zeros((n, 6), dtype=np.uint64)
+    plies = np.zeros(n, dtype=np.uint32)
+    _check(lib().pg_random_positions(seed, n, min_ply, max_ply, _ptr(arr), _ptr(plies)), "pg_random_positions")
+    return arr, plies
+
+
+def positions_array(positions):
+    """list of Pos48 / FEN strings / (n,6) array -> contiguous (n,6) uint64 array."""
+    if isinstance(positions, np.ndarray):
+        return np.ascontiguousarray(positions, dtype=np.uint64).reshape(-1, 6)
+    out = np.zeros((len(positions), 6), dtype=np.uint64)
+    for i, p in enumerate(positions):
+        if isinstance(p, str):
+            p = read_fen_string(p)
+        out[i] = p.words()
+    return out
+
+
+def pos_from_words(w):
+    return Pos48(*[int(x) for x in w])
+
+
+# ---------------------------------------------------------------- device side
+class Device:
+    """One GPU context (pg_ctx): tables uploaded, tree arena reserved."""
+
+    def __init__(self, device=0, arena_bytes=0, slider=SLIDER_AUTO, timing=False, blocks_per_cu=0):
+        L = lib()
+        cfg = Config(arena_bytes, slider, 0, blocks_per_cu, 1 if timing else 0)
+        h = ctypes.c_void_p()
+        _check(L.pg_init(device, ctypes.byref(cfg), ctypes.byref(h)), "pg_init")
+        self._h = h
+        self.device = device
+
+    def close(self):
+        if self._h:
+            lib().pg_shutdown(self._h)
+            self._h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    @property
+    def config(self):
+        c = Config()
+        _check(lib().pg_get_config(self._h, ctypes.byref(c)), "pg_get_config")
+        return {n: getattr(c, n) for n, _ in c._fields_}
+
+    def set_slider(self, slider):
+        _check(lib().pg_set_slider(self._h, slider), "pg_set_slider")
+
+    def perft(self, pos, depth, slider=SLIDER_AUTO):
+        if isinstance(pos, str):
+            pos = read_fen_string(pos)
+        out = u64()
+        _check(lib().pg_perft(self._h, ctypes.byref(pos), depth, slider, ctypes.byref(out)), "pg_perft")
+        return int(out.value)
+
+    def last_stats(self):
+        s = Stats()
+        _check(lib().pg_last_stats(self._h, ctypes.byref(s)), "pg_last_stats")
+        return s.as_dict()
+
+    def count_moves(self, positions, slider=SLIDER_AUTO):
+        arr = positions_array(positions)
+        out = np.zeros(len(arr), dtype=np.uint32)
+        _check(lib().pg_device_count_moves(self._h, _ptr(arr), len(arr), slider, _ptr(out)), "pg_device_count_moves")
+        return out
+
+    def generate_moves(self, positions, slider=SLIDER_AUTO):
+        arr = positions_array(positions)
+        moves = np.zeros((len(arr), 256), dtype=np.uint16)
+        cnt = np.zeros(len(arr), dtype=np.uint32)
+        _check(lib().pg_device_generate_moves(self._h, _ptr(arr), len(arr), slider, _ptr(moves), _ptr(cnt)),
+               "pg_device_generate_moves")
+        return [list(moves[i, :cnt[i]]) for i in range(len(arr))]
+
+    def make_moves(self, positions, moves, keys=None):
+        arr = positions_array(positions)
+        mv = np.ascontiguousarray(moves, dtype=np.uint16)
+        out = np.zeros_like(arr)
+        k = None if keys is None else np.ascontiguousarray(keys, dtype=np.uint64).reshape(-1, 2).copy()
+        _check(lib().pg_device_make_move(self._h, _ptr(arr), _ptr(mv), len(arr), _ptr(out),
+                                         _ptr(k) if k is not None else None), "pg_device_make_move")
+        return out, k
+
+    def slider_attacks(self, slider, rook, squares, occupancy):
+        sq = np.ascontiguousarray(squares, dtype=np.uint8)
+        occ = np.ascontiguousarray(occupancy, dtype=np.uint64)
+        out = np.zeros(len(sq), dtype=np.uint64)
+        _check(lib().pg_device_slider_attacks(self._h, slider, int(rook), _ptr(sq), _ptr(occ), len(sq), _ptr(out)),
+               "pg_device_slider_attacks")
+        return out
+
+
+def init_gpu(device=0, **kw):
+    """initGPU (launcher.h:1506): select the GPU, upload tables, reserve the arena."""
+    return Device(device, **kw)
+
+
+def perft_launcher(dev, pos, depth, launch_depth=None, slider=SLIDER_AUTO, verbose=True):
+    """perftLauncher, no-TT branch (launcher.h:1455-1503): run the GPU perft and
+    print the reference's result line.  launch_depth is accepted for call
+    compatibility; the level loop sizes its own splits from the arena."""
+    t0 = time.perf_counter()
+    res = dev.perft(pos, depth, slider)
+    dt = time.perf_counter() - t0
+    if verbose:
+        print("\nGPU Perft %d: %d,   Time taken: %g seconds, nps: %d" % (depth, res, dt, int(res / dt) if dt > 0 else 0))
+    return res

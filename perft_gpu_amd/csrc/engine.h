@@ -1,0 +1,39 @@
+// engine.h -- per-device perft context (the object behind pg_ctx).
+#pragma once
+#include <hip/hip_runtime.h>
+#include "../../include/perft_gpu.h"
+#include "tables.h"
+
+namespace pg {
+
+void set_error(const char *fmt, ...);
+
+struct Engine {
+    int device = 0, cus = 0, bpc = 8, slider = SL_KOGGE;
+    size_t ldsPerBlock = 0;
+    int clockKHz = 0;
+    bool timing = false;
+    pg_config cfg{};
+    hipStream_t stream = nullptr;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    u8 *dTables = nullptr;
+    Tables dt{};
+    u8 *arena = nullptr;     // tree arena, used as a stack
+    size_t arenaBytes = 0, top = 0, peak = 0;
+    u8 *dSmall = nullptr;    // accumulator and error flag
+    u64 *dAcc = nullptr;
+    u32 *dErr = nullptr;
+    u64 *hostBuf = nullptr;  // pinned
+    pg_stats stats{};
+
+    void *alloc(size_t bytes);
+    u32 grid(u64 n, int wg) const;
+    int read_u64(const u64 *d, u64 *h);
+    int scan(const u32 *cnt, u64 n, u64 *off, u64 *total);
+    int leaf(int v, const Pos *b, u64 n, const u32 *par, const u16 *mv, u64 m);
+    int bfs(int v, const Pos *boards, u64 n, const u32 *cnt, int r, int ply);
+    int split(int v, const Pos *boards, u64 n, const u32 *cnt, int r, int ply);
+    int perft(const Pos &root, u32 depth, int v, u64 *out);
+};
+
+}  // namespace pg

@@ -1,0 +1,455 @@
+// engine.hip -- device context, tree arena and the breadth-first perft
+// driver behind the C ABI (include/perft_gpu.h).
+//
+// Replaces the reference's CDP drivers (perft_bb_driver_gpu /
+// perft_bb_gpu_simple, perft_bb.h:1027-1220) with a host-driven level loop on
+// one HIP stream.  The tree lives in one pre-reserved HBM arena used as a
+// stack (the reference bump-allocates a 3 GiB buffer with 32-bit offsets,
+// perft_bb.h:113-139); a level whose children do not fit is split in halves
+// and each half expanded on its own (the reference instead fixes a launch
+// depth, perft.cu:117-120).
+#include <hip/hip_runtime.h>
+#include <chrono>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <vector>
+#include "../../include/perft_gpu.h"
+#include "engine.h"
+#include "kernels.h"
+
+using namespace pg;
+
+namespace pg {
+
+thread_local char g_last_error[512];
+void set_error(const char *fmt, ...) {
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(g_last_error, sizeof(g_last_error), fmt, ap);
+    va_end(ap);
+}
+
+#define HIP_TRY(expr)                                                   \
+    do {                                                                \
+        hipError_t e_ = (expr);                                         \
+        if (e_ != hipSuccess) {                                         \
+            set_error("HIP error %s at line %d", hipGetErrorString(e_), __LINE__); \
+            return PG_EHIP;                                             \
+        }                                                               \
+    } while (0)
+
+static double now_ms() {
+    return std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now().time_since_epoch()).count();
+}
+
+void *Engine::alloc(size_t bytes) {
+    size_t a = (top + 255) & ~(size_t)255;
+    if (a + bytes > arenaBytes) return nullptr;
+    top = a + bytes;
+    if (top > peak) peak = top;
+    return arena + a;
+}
+
+u32 Engine::grid(u64 n, int wg) const {
+    u64 want = (n + wg - 1) / wg;
+    u64 cap = (u64)cus * (wg >= 1024 ? 1 : (u64)bpc);
+    if (want > cap) want = cap;
+    return (u32)(want ? want : 1);
+}
+
+int Engine::read_u64(const u64 *d, u64 *h) {
+    HIP_TRY(hipMemcpyAsync(hostBuf, d, sizeof(u64), hipMemcpyDeviceToHost, stream));
+    HIP_TRY(hipStreamSynchronize(stream));
+    *h = *hostBuf;
+    return PG_OK;
+}
+
+// ---------------------------------------------------------------- dispatch
+template <int V> static void launch_count(Engine &c, const Pos *b, u64 n, u32 *cnt) {
+    constexpr int W = wg_size<V>();
+    k_count<V><<<c.grid(n, W), W, 0, c.stream>>>(b, n, cnt, c.dt);
+}
+template <int V> static void launch_count_sum(Engine &c, const Pos *b, u64 n, u64 *acc) {
+    constexpr int W = wg_size<V>();
+    k_count_sum<V><<<c.grid(n, W), W, 0, c.stream>>>(b, n, acc, c.dt);
+}
+template <int V> static void launch_generate(Engine &c, const Pos *b, u64 n, const u64 *off, u16 *mv, u32 *par) {
+    constexpr int W = wg_size<V>();
+    k_generate<V><<<c.grid(n, W), W, 0, c.stream>>>(b, n, off, mv, par, c.dErr, c.dt);
+}
+template <int V> static void launch_expand(Engine &c, const Pos *b, const u32 *par, const u16 *mv, u64 m, Pos *ch, u32 *cnt) {
+    constexpr int W = wg_size<V>();
+    k_expand<V><<<c.grid(m, W), W, 0, c.stream>>>(b, par, mv, m, ch, cnt, c.dt);
+}
+template <int V> static void launch_leaf(Engine &c, const Pos *b, const u32 *par, const u16 *mv, u64 m, u64 *acc) {
+    constexpr int W = wg_size<V>();
+    k_leaf<V><<<c.grid(m, W), W, 0, c.stream>>>(b, par, mv, m, acc, c.dt);
+}
+
+#define PG_VARIANT_CALL(v, fn, ...)                  \
+    switch (v) {                                     \
+    case SL_KOGGE: fn<SL_KOGGE>(__VA_ARGS__); break; \
+    case SL_PLAIN: fn<SL_PLAIN>(__VA_ARGS__); break; \
+    case SL_FANCY: fn<SL_FANCY>(__VA_ARGS__); break; \
+    default: fn<SL_BYTE>(__VA_ARGS__); break;        \
+    }
+
+int Engine::scan(const u32 *cnt, u64 n, u64 *off, u64 *total) {
+    const u64 tiles = (n + SCAN_TILE - 1) / SCAN_TILE;
+    const size_t mark = top;
+    u64 *tileSum = (u64 *)alloc(tiles * sizeof(u64));
+    if (!tileSum) return PG_ENOMEM;
+    k_scan_reduce<<<(u32)tiles, SCAN_WG, 0, stream>>>(cnt, n, tileSum);
+    k_scan_spine<<<1, 1024, 0, stream>>>(tileSum, tiles, off + n);
+    k_scan_down<<<(u32)tiles, SCAN_WG, 0, stream>>>(cnt, n, tileSum, off);
+    HIP_TRY(hipGetLastError());
+    int rc = read_u64(off + n, total);
+    top = mark;
+    return rc;
+}
+
+int Engine::leaf(int v, const Pos *b, u64 n, const u32 *par, const u16 *mv, u64 m) {
+    if (timing) HIP_TRY(hipEventRecord(ev0, stream));
+    PG_VARIANT_CALL(v, launch_leaf, *this, b, par, mv, m, dAcc);
+    HIP_TRY(hipGetLastError());
+    if (timing) {
+        HIP_TRY(hipEventRecord(ev1, stream));
+        HIP_TRY(hipEventSynchronize(ev1));
+        float ms = 0;
+        HIP_TRY(hipEventElapsedTime(&ms, ev0, ev1));
+        stats.leaf_kernel_ms += ms;
+    }
+    stats.leaf_launches++;
+    stats.leaf_items += m;
+    stats.leaf_parents += n;
+    return PG_OK;
+}
+
+// Sum of perft(r) over n boards that sit at `ply` below the root.  cnt, when
+// given, already holds each board's move count.
+int Engine::bfs(int v, const Pos *boards, u64 n, const u32 *cnt, int r, int ply) {
+    if (n == 0) return PG_OK;
+    if (r == 1) {
+        if (cnt) k_sum_u32<<<grid(n, 256), 256, 0, stream>>>(cnt, n, dAcc);
+        else PG_VARIANT_CALL(v, launch_count_sum, *this, boards, n, dAcc);
+        HIP_TRY(hipGetLastError());
+        return PG_OK;
+    }
+    if (n > 0xFFFFFFFFULL) return split(v, boards, n, cnt, r, ply);
+    const size_t mark = top;
+    u32 *c = const_cast<u32 *>(cnt);
+    if (!c) {
+        c = (u32 *)alloc(n * sizeof(u32));
+        if (!c) { top = mark; return split(v, boards, n, cnt, r, ply); }
+        PG_VARIANT_CALL(v, launch_count, *this, boards, n, c);
+    }
+    u64 *off = (u64 *)alloc((n + 1) * sizeof(u64));
+    u64 total = 0;
+    int rc = off ? scan(c, n, off, &total) : PG_ENOMEM;
+    if (rc == PG_ENOMEM) { top = mark; return split(v, boards, n, cnt, r, ply); }
+    if (rc) return rc;
+    if (ply + 1 < 16) stats.nodes[ply + 1] += total;
+    if (total == 0) { top = mark; return PG_OK; }
+
+    Pos *children = nullptr;
+    u32 *ccnt = nullptr;
+    if (r > 2) {
+        children = (Pos *)alloc(total * sizeof(Pos));
+        ccnt = children ? (u32 *)alloc(total * sizeof(u32)) : nullptr;
+    }
+    u16 *mv = (r == 2 || ccnt) ? (u16 *)alloc(total * sizeof(u16)) : nullptr;
+    u32 *par = mv ? (u32 *)alloc(total * sizeof(u32)) : nullptr;
+    if (!par) {
+        top = mark;
+        if (n == 1) { set_error("one board's children do not fit the arena (ply %d)", ply); return PG_ENOMEM; }
+        return split(v, boards, n, cnt, r, ply);
+    }
+    PG_VARIANT_CALL(v, launch_generate, *this, boards, n, off, mv, par);
+    HIP_TRY(hipGetLastError());
+    if (r == 2) {
+        rc = leaf(v, boards, n, par, mv, total);
+        top = mark;
+        return rc;
+    }
+    PG_VARIANT_CALL(v, launch_expand, *this, boards, par, mv, total, children, ccnt);
+    HIP_TRY(hipGetLastError());
+    top = (size_t)((u8 *)ccnt - arena) + total * sizeof(u32);  // drop moves and parents
+    rc = bfs(v, children, total, ccnt, r - 1, ply + 1);
+    top = mark;
+    return rc;
+}
+
+int Engine::split(int v, const Pos *boards, u64 n, const u32 *cnt, int r, int ply) {
+    if (n <= 1) {
+        set_error("arena too small for a single board at ply %d", ply);
+        return PG_ENOMEM;
+    }
+    stats.splits++;
+    const u64 h = n / 2;
+    int rc = bfs(v, boards, h, cnt, r, ply);
+    if (rc) return rc;
+    return bfs(v, boards + h, n - h, cnt ? cnt + h : nullptr, r, ply);
+}
+
+int Engine::perft(const Pos &root, u32 depth, int v, u64 *out) {
+    memset(&stats, 0, sizeof(stats));
+    const double t0 = now_ms();
+    top = 0;
+    peak = 0;
+    if (depth == 0) { *out = 1; return PG_OK; }
+    Pos *d = (Pos *)alloc(sizeof(Pos));
+    if (!d) return PG_ENOMEM;
+    HIP_TRY(hipMemcpyAsync(d, &root, sizeof(Pos), hipMemcpyHostToDevice, stream));
+    HIP_TRY(hipMemsetAsync(dAcc, 0, sizeof(u64), stream));
+    HIP_TRY(hipMemsetAsync(dErr, 0, sizeof(u32), stream));
+    int rc = bfs(v, d, 1, nullptr, (int)depth, 0);
+    if (rc) return rc;
+    u64 res = 0;
+    rc = read_u64(dAcc, &res);
+    if (rc) return rc;
+    u32 err = 0;
+    HIP_TRY(hipMemcpy(&err, dErr, sizeof(u32), hipMemcpyDeviceToHost));
+    if (err) {
+        set_error("move generator count/generate mismatch (flag %u)", err);
+        return PG_EINVAL;
+    }
+    stats.leaves = res;
+    stats.peak_arena = peak;
+    stats.total_ms = now_ms() - t0;
+    *out = res;
+    return PG_OK;
+}
+
+}  // namespace pg
+
+// ====================================================================== C ABI
+struct pg_ctx : public pg::Engine {};
+
+static int resolve_slider(const pg_ctx *c, int s) {
+    if (s == PG_SLIDER_AUTO) return c->slider;
+    if (s < 0 || s >= SL_COUNT) return -1;
+    return s;
+}
+
+// ------------------------------------------------------------------ probes
+template <int V> static void launch_probe_gen(Engine &c, const Pos *b, u64 n, u16 *mv, u32 *cnt) {
+    constexpr int W = wg_size<V>();
+    k_probe_generate<V><<<c.grid(n, W), W, 0, c.stream>>>(b, n, mv, cnt, c.dt);
+}
+template <int V> static void launch_probe_slider(Engine &c, int rook, const u8 *sq, const u64 *occ, u64 n, u64 *out) {
+    constexpr int W = wg_size<V>();
+    k_probe_slider<V><<<c.grid(n, W), W, 0, c.stream>>>(rook, sq, occ, n, out, c.dt);
+}
+
+extern "C" {
+
+int pg_init(int device, const pg_config *cfg, pg_ctx **out) {
+    if (!out) return PG_EINVAL;
+    *out = nullptr;
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) {
+        set_error("no GPU visible (%d devices)", ndev);
+        return PG_ENODEV;
+    }
+    if (device < 0 || device >= ndev) return PG_EINVAL;
+    const char *terr = nullptr;
+    const HostTables *ht = host_tables(&terr);
+    if (!ht) {
+        set_error("table build failed: %s", terr ? terr : "?");
+        return PG_ETABLE;
+    }
+    pg_ctx *c = new pg_ctx();
+    c->device = device;
+    pg_config def = {0, PG_SLIDER_AUTO, 0, 0, 0};
+    c->cfg = cfg ? *cfg : def;
+    c->slider = (c->cfg.slider >= 0 && c->cfg.slider < SL_COUNT) ? c->cfg.slider : SL_KOGGE;
+    c->bpc = c->cfg.blocks_per_cu > 0 ? c->cfg.blocks_per_cu : 8;
+    c->timing = c->cfg.timing != 0;
+    auto fail = [&](int rc) { pg_shutdown(c); return rc; };
+    if (hipSetDevice(device) != hipSuccess) return fail(PG_EHIP);
+    hipDeviceProp_t prop;
+    if (hipGetDeviceProperties(&prop, device) != hipSuccess) return fail(PG_EHIP);
+    c->cus = prop.multiProcessorCount;
+    c->ldsPerBlock = prop.sharedMemPerBlock;
+    c->clockKHz = prop.clockRate;
+    if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) return fail(PG_EHIP);
+    if (hipEventCreate(&c->ev0) != hipSuccess || hipEventCreate(&c->ev1) != hipSuccess) return fail(PG_EHIP);
+    std::vector<u8> blob;
+    pack_tables(*ht, blob);
+    if (hipMalloc(&c->dTables, blob.size()) != hipSuccess) return fail(PG_ENOMEM);
+    if (hipMemcpy(c->dTables, blob.data(), blob.size(), hipMemcpyHostToDevice) != hipSuccess) return fail(PG_EHIP);
+    c->dt = tables_at(c->dTables);
+    if (hipMalloc(&c->dSmall, 256) != hipSuccess) return fail(PG_ENOMEM);
+    c->dAcc = (u64 *)c->dSmall;
+    c->dErr = (u32 *)(c->dSmall + 64);
+    if (hipHostMalloc(&c->hostBuf, 64) != hipSuccess) return fail(PG_ENOMEM);
+    size_t freeB = 0, totalB = 0;
+    (void)hipMemGetInfo(&freeB, &totalB);
+    size_t want = c->cfg.arena_bytes ? c->cfg.arena_bytes : (48ull << 30);
+    if (!c->cfg.arena_bytes && want > freeB / 2) want = freeB / 2;
+    if (want < (64ull << 20)) want = 64ull << 20;
+    if (hipMalloc(&c->arena, want) != hipSuccess) {
+        set_error("arena reservation of %zu MiB failed", want >> 20);
+        return fail(PG_ENOMEM);
+    }
+    c->arenaBytes = want;
+    c->cfg.arena_bytes = want;
+    c->cfg.slider = c->slider;
+    *out = c;
+    return PG_OK;
+}
+
+void pg_shutdown(pg_ctx *c) {
+    if (!c) return;
+    (void)hipSetDevice(c->device);
+    if (c->stream) (void)hipStreamSynchronize(c->stream);
+    if (c->arena) (void)hipFree(c->arena);
+    if (c->dTables) (void)hipFree(c->dTables);
+    if (c->dSmall) (void)hipFree(c->dSmall);
+    if (c->hostBuf) (void)hipHostFree(c->hostBuf);
+    if (c->ev0) (void)hipEventDestroy(c->ev0);
+    if (c->ev1) (void)hipEventDestroy(c->ev1);
+    if (c->stream) (void)hipStreamDestroy(c->stream);
+    delete c;
+}
+
+int pg_get_config(pg_ctx *c, pg_config *out) {
+    if (!c || !out) return PG_EINVAL;
+    *out = c->cfg;
+    return PG_OK;
+}
+
+int pg_set_slider(pg_ctx *c, int s) {
+    if (!c || s < 0 || s >= SL_COUNT) return PG_EINVAL;
+    c->slider = s;
+    c->cfg.slider = s;
+    return PG_OK;
+}
+
+int pg_perft(pg_ctx *c, const pg_pos48 *pos, uint32_t depth, int slider, uint64_t *out) {
+    if (!c || !pos || !out || depth > 15) return PG_EINVAL;
+    const int v = resolve_slider(c, slider);
+    if (v < 0) return PG_EINVAL;
+    if (hipSetDevice(c->device) != hipSuccess) return PG_EHIP;
+    Pos p;
+    memcpy(&p, pos, sizeof(p));
+    if (!pos_sane(p)) {
+        set_error("position needs exactly one king per side");
+        return PG_EINVAL;
+    }
+    return c->perft(p, depth, v, reinterpret_cast<u64 *>(out));
+}
+
+int pg_last_stats(pg_ctx *c, pg_stats *out) {
+    if (!c || !out) return PG_EINVAL;
+    *out = c->stats;
+    return PG_OK;
+}
+
+const char *pg_last_error(void) { return g_last_error; }
+
+static bool boards_sane(const pg_pos48 *pos, uint32_t n) {
+    for (uint32_t i = 0; i < n; i++) {
+        Pos p;
+        memcpy(&p, &pos[i], sizeof(p));
+        if (!pos_sane(p)) {
+            set_error("board %u needs exactly one king per side", i);
+            return false;
+        }
+    }
+    return true;
+}
+
+static int stage_in(pg_ctx *c, const void *h, size_t bytes, void **d) {
+    *d = c->alloc(bytes);
+    if (!*d) return PG_ENOMEM;
+    if (h && hipMemcpyAsync(*d, h, bytes, hipMemcpyHostToDevice, c->stream) != hipSuccess) return PG_EHIP;
+    return PG_OK;
+}
+static int stage_out(pg_ctx *c, void *h, const void *d, size_t bytes) {
+    if (hipMemcpyAsync(h, d, bytes, hipMemcpyDeviceToHost, c->stream) != hipSuccess) return PG_EHIP;
+    if (hipStreamSynchronize(c->stream) != hipSuccess) return PG_EHIP;
+    return PG_OK;
+}
+
+#define PROBE_TRY(x) do { int r_ = (x); if (r_) { c->top = 0; return r_; } } while (0)
+
+int pg_device_count_moves(pg_ctx *c, const pg_pos48 *pos, uint32_t n, int slider, uint32_t *out) {
+    if (!c || (!pos && n) || (!out && n)) return PG_EINVAL;
+    const int v = resolve_slider(c, slider);
+    if (v < 0) return PG_EINVAL;
+    if (!boards_sane(pos, n)) return PG_EINVAL;
+    if (!n) return PG_OK;
+    (void)hipSetDevice(c->device);
+    c->top = 0;
+    void *db, *dc;
+    PROBE_TRY(stage_in(c, pos, n * sizeof(Pos), &db));
+    PROBE_TRY(stage_in(c, nullptr, n * sizeof(u32), &dc));
+    PG_VARIANT_CALL(v, launch_count, *c, (const Pos *)db, n, (u32 *)dc);
+    if (hipGetLastError() != hipSuccess) return PG_EHIP;
+    PROBE_TRY(stage_out(c, out, dc, n * sizeof(u32)));
+    c->top = 0;
+    return PG_OK;
+}
+
+int pg_device_generate_moves(pg_ctx *c, const pg_pos48 *pos, uint32_t n, int slider, pg_move *moves, uint32_t *counts) {
+    if (!c || (n && (!pos || !moves || !counts))) return PG_EINVAL;
+    const int v = resolve_slider(c, slider);
+    if (v < 0) return PG_EINVAL;
+    if (!boards_sane(pos, n)) return PG_EINVAL;
+    if (!n) return PG_OK;
+    (void)hipSetDevice(c->device);
+    c->top = 0;
+    void *db, *dm, *dc;
+    PROBE_TRY(stage_in(c, pos, n * sizeof(Pos), &db));
+    PROBE_TRY(stage_in(c, nullptr, (size_t)n * 256 * sizeof(u16), &dm));
+    PROBE_TRY(stage_in(c, nullptr, n * sizeof(u32), &dc));
+    PG_VARIANT_CALL(v, launch_probe_gen, *c, (const Pos *)db, n, (u16 *)dm, (u32 *)dc);
+    if (hipGetLastError() != hipSuccess) return PG_EHIP;
+    PROBE_TRY(stage_out(c, moves, dm, (size_t)n * 256 * sizeof(u16)));
+    PROBE_TRY(stage_out(c, counts, dc, n * sizeof(u32)));
+    c->top = 0;
+    return PG_OK;
+}
+
+int pg_device_make_move(pg_ctx *c, const pg_pos48 *pos, const pg_move *moves, uint32_t n, pg_pos48 *out, pg_key128 *keys) {
+    if (!c || (n && (!pos || !moves || !out))) return PG_EINVAL;
+    if (!boards_sane(pos, n)) return PG_EINVAL;
+    if (!n) return PG_OK;
+    (void)hipSetDevice(c->device);
+    c->top = 0;
+    void *db, *dm, *dout, *dk = nullptr;
+    PROBE_TRY(stage_in(c, pos, n * sizeof(Pos), &db));
+    PROBE_TRY(stage_in(c, moves, n * sizeof(u16), &dm));
+    PROBE_TRY(stage_in(c, nullptr, n * sizeof(Pos), &dout));
+    if (keys) PROBE_TRY(stage_in(c, keys, n * sizeof(Key128), &dk));
+    k_probe_make<<<c->grid(n, 256), 256, 0, c->stream>>>((const Pos *)db, (const u16 *)dm, n, (Pos *)dout, (Key128 *)dk, c->dt);
+    if (hipGetLastError() != hipSuccess) return PG_EHIP;
+    PROBE_TRY(stage_out(c, out, dout, n * sizeof(Pos)));
+    if (keys) PROBE_TRY(stage_out(c, keys, dk, n * sizeof(Key128)));
+    c->top = 0;
+    return PG_OK;
+}
+
+int pg_device_slider_attacks(pg_ctx *c, int slider, int rook, const uint8_t *sq, const uint64_t *occ, uint32_t n, uint64_t *out) {
+    if (!c || (n && (!sq || !occ || !out))) return PG_EINVAL;
+    const int v = resolve_slider(c, slider);
+    if (v < 0) return PG_EINVAL;
+    for (uint32_t i = 0; i < n; i++)
+        if (sq[i] > 63) return PG_EINVAL;
+    if (!n) return PG_OK;
+    (void)hipSetDevice(c->device);
+    c->top = 0;
+    void *ds, *dO, *dout;
+    PROBE_TRY(stage_in(c, sq, n, &ds));
+    PROBE_TRY(stage_in(c, occ, n * sizeof(u64), &dO));
+    PROBE_TRY(stage_in(c, nullptr, n * sizeof(u64), &dout));
+    PG_VARIANT_CALL(v, launch_probe_slider, *c, rook, (const u8 *)ds, (const u64 *)dO, n, (u64 *)dout);
+    if (hipGetLastError() != hipSuccess) return PG_EHIP;
+    PROBE_TRY(stage_out(c, out, dout, n * sizeof(u64)));
+    c->top = 0;
+    return PG_OK;
+}
+
+}  // extern "C"

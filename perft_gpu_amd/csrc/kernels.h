@@ -1,0 +1,264 @@
+// kernels.h -- gfx950 kernels of the breadth-first perft.
+//
+// One level of the tree is expanded by four kernels (the reference's level
+// loop, perft_bb.h:1071-1135, without device-side launches):
+//   k_count     moves per board                        (countMoves)
+//   scan        exclusive prefix sum of the counts     (ScanD, scan.cuh:329)
+//   k_generate  move list + parent index per child     (generateMoves +
+//               IntervalExpandDGenValues folded together, intervalmove.cuh:404)
+//   k_expand    child board = make(parent, move), and the child's move count
+//               (makemove_and_count_moves_single_level, perft_bb.h:601)
+// and the last two plies by
+//   k_leaf      sum over (parent, move) of countMoves(make(parent, move))
+//               (makeMove_and_perft_single_level_indices, perft_bb.h:412)
+// All kernels are grid-stride loops sized to the CU count, so one launch
+// covers a level of any size; indices are 64-bit.
+#pragma once
+#include "movegen.h"
+
+namespace pg {
+
+constexpr int kByteLdsBytes = FANCY_TABLE_SIZE + (BYTE_ROOK_SIZE + BYTE_BISHOP_SIZE) * 8 + 2 * 64 * 16 + 2 * 64 * 8;
+
+__device__ __forceinline__ u64 wave_sum(u64 v) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    return v;
+}
+
+template <class T>
+__device__ __forceinline__ void lds_copy(T *dst, const T *src, int count) {
+    // 16-byte chunks; every table here is a multiple of 16 bytes
+    const uint4 *s = reinterpret_cast<const uint4 *>(src);
+    uint4 *d = reinterpret_cast<uint4 *>(dst);
+    const int chunks = (int)(count * sizeof(T) / 16);
+    for (int i = threadIdx.x; i < chunks; i += blockDim.x) d[i] = s[i];
+}
+
+// The byte-lookup variant keeps its whole working set (147 KiB) in LDS; the
+// other variants read their tables through the L1/L2 caches.
+template <int V>
+__device__ __forceinline__ void stage_tables(Tables &t) {
+    if constexpr (V == SL_BYTE) {
+        __shared__ __align__(16) u8 sIdx[FANCY_TABLE_SIZE];
+        __shared__ __align__(16) u64 sRook[BYTE_ROOK_SIZE];
+        __shared__ __align__(16) u64 sBishop[BYTE_BISHOP_SIZE];
+        __shared__ __align__(16) FancyEntry sRF[64], sBF[64];
+        __shared__ __align__(16) u64 sRM[64], sBM[64];
+        lds_copy(sIdx, t.byteIdx, FANCY_TABLE_SIZE);
+        lds_copy(sRook, t.byteRook, BYTE_ROOK_SIZE);
+        lds_copy(sBishop, t.byteBishop, BYTE_BISHOP_SIZE);
+        lds_copy(sRF, t.rookFancy, 64);
+        lds_copy(sBF, t.bishopFancy, 64);
+        lds_copy(sRM, t.rookMask, 64);
+        lds_copy(sBM, t.bishopMask, 64);
+        __syncthreads();
+        t.byteIdx = sIdx;
+        t.byteRook = sRook;
+        t.byteBishop = sBishop;
+        t.rookFancy = sRF;
+        t.bishopFancy = sBF;
+        t.rookMask = sRM;
+        t.bishopMask = sBM;
+    }
+}
+
+template <int V> constexpr int wg_size() { return V == SL_BYTE ? 1024 : 256; }
+
+#define PG_GRID_LOOP(i, n) \
+    for (u64 i = (u64)blockIdx.x * blockDim.x + threadIdx.x; i < (n); i += (u64)gridDim.x * blockDim.x)
+
+// number of legal moves of a board after one move, side-resolved once
+template <int V>
+__device__ __forceinline__ u32 count_after(Pos p, u16 mv, const Tables &t) {
+    if (pos_side(p)) {
+        make_move<BLACK, false>(p, mv, nullptr, t);
+        return count_moves<WHITE, V>(p, t);
+    }
+    make_move<WHITE, false>(p, mv, nullptr, t);
+    return count_moves<BLACK, V>(p, t);
+}
+
+template <int V>
+__global__ __launch_bounds__(wg_size<V>()) void k_count(const Pos *__restrict__ b, u64 n, u32 *__restrict__ cnt, Tables t) {
+    stage_tables<V>(t);
+    PG_GRID_LOOP(i, n) cnt[i] = count_any<V>(b[i], t);
+}
+
+template <int V>
+__global__ __launch_bounds__(wg_size<V>()) void k_count_sum(const Pos *__restrict__ b, u64 n, u64 *acc, Tables t) {
+    stage_tables<V>(t);
+    u64 s = 0;
+    PG_GRID_LOOP(i, n) s += count_any<V>(b[i], t);
+    s = wave_sum(s);
+    if ((threadIdx.x & 63) == 0 && s) atomicAdd(acc, s);
+}
+
+__global__ __launch_bounds__(256) void k_sum_u32(const u32 *__restrict__ c, u64 n, u64 *acc) {
+    u64 s = 0;
+    PG_GRID_LOOP(i, n) s += c[i];
+    s = wave_sum(s);
+    if ((threadIdx.x & 63) == 0 && s) atomicAdd(acc, s);
+}
+
+// err: set to 1 if a board generates a different number of moves than it
+// counted (the two must agree for the level layout to be valid).
+template <int V>
+__global__ __launch_bounds__(wg_size<V>()) void k_generate(const Pos *__restrict__ b, u64 n, const u64 *__restrict__ off,
+                                                           u16 *__restrict__ moves, u32 *__restrict__ parent, u32 *err,
+                                                           Tables t) {
+    stage_tables<V>(t);
+    PG_GRID_LOOP(i, n) {
+        const u64 o = off[i];
+        const u32 room = (u32)(off[i + 1] - o);
+        const u32 c = generate_any<V>(b[i], t, moves + o, room);
+        if (c != room) *err = 1;
+        for (u32 j = 0; j < room; j++) parent[o + j] = (u32)i;
+    }
+}
+
+template <int V>
+__global__ __launch_bounds__(wg_size<V>()) void k_expand(const Pos *__restrict__ parents, const u32 *__restrict__ pidx,
+                                                         const u16 *__restrict__ moves, u64 m, Pos *__restrict__ children,
+                                                         u32 *__restrict__ cnt, Tables t) {
+    stage_tables<V>(t);
+    PG_GRID_LOOP(i, m) {
+        Pos p = parents[pidx[i]];
+        make_any(p, moves[i], t);
+        children[i] = p;
+        cnt[i] = count_any<V>(p, t);
+    }
+}
+
+template <int V>
+__global__ __launch_bounds__(wg_size<V>()) void k_leaf(const Pos *__restrict__ parents, const u32 *__restrict__ pidx,
+                                                       const u16 *__restrict__ moves, u64 m, u64 *acc, Tables t) {
+    stage_tables<V>(t);
+    u64 s = 0;
+    PG_GRID_LOOP(i, m) s += count_after<V>(parents[pidx[i]], moves[i], t);
+    s = wave_sum(s);
+    if ((threadIdx.x & 63) == 0 && s) atomicAdd(acc, s);
+}
+
+// ---------------------------------------------------------------- scan
+// Exclusive scan of u32 counts into u64 offsets; off[n] receives the total.
+// Reduce-then-scan over 2048-item tiles: k_scan_reduce (tile sums),
+// k_scan_spine (one work-group scans the tile sums), k_scan_down (each tile
+// rescans itself from its base).
+constexpr int SCAN_WG = 256, SCAN_IPT = 8, SCAN_TILE = SCAN_WG * SCAN_IPT;
+
+__device__ __forceinline__ u64 wave_incl_scan(u64 v) {
+    const int lane = threadIdx.x & 63;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+        const u64 y = __shfl_up(v, o, 64);
+        if (lane >= o) v += y;
+    }
+    return v;
+}
+
+// exclusive scan across the work-group; *total gets the group sum
+template <int WG>
+__device__ __forceinline__ u64 block_excl_scan(u64 v, u64 *total) {
+    __shared__ u64 waveTot[WG / 64];
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const u64 incl = wave_incl_scan(v);
+    if (lane == 63) waveTot[w] = incl;
+    __syncthreads();
+    u64 base = 0, tot = 0;
+#pragma unroll
+    for (int i = 0; i < WG / 64; i++) {
+        const u64 x = waveTot[i];
+        if (i < w) base += x;
+        tot += x;
+    }
+    __syncthreads();
+    *total = tot;
+    return base + incl - v;
+}
+
+__device__ __forceinline__ void load_tile(const u32 *c, u64 n, u64 base, u32 v[SCAN_IPT]) {
+    if (base + SCAN_IPT <= n && (reinterpret_cast<uintptr_t>(c + base) & 15) == 0) {
+        const uint4 a = *reinterpret_cast<const uint4 *>(c + base);
+        const uint4 b = *reinterpret_cast<const uint4 *>(c + base + 4);
+        v[0] = a.x; v[1] = a.y; v[2] = a.z; v[3] = a.w;
+        v[4] = b.x; v[5] = b.y; v[6] = b.z; v[7] = b.w;
+    } else {
+#pragma unroll
+        for (int j = 0; j < SCAN_IPT; j++) v[j] = base + j < n ? c[base + j] : 0;
+    }
+}
+
+__global__ __launch_bounds__(SCAN_WG) void k_scan_reduce(const u32 *__restrict__ c, u64 n, u64 *__restrict__ tileSum) {
+    const u64 base = (u64)blockIdx.x * SCAN_TILE + (u64)threadIdx.x * SCAN_IPT;
+    u32 v[SCAN_IPT];
+    load_tile(c, n, base, v);
+    u64 s = 0;
+#pragma unroll
+    for (int j = 0; j < SCAN_IPT; j++) s += v[j];
+    u64 tot;
+    block_excl_scan<SCAN_WG>(s, &tot);
+    if (threadIdx.x == 0) tileSum[blockIdx.x] = tot;
+}
+
+__global__ __launch_bounds__(1024) void k_scan_spine(u64 *tileSum, u64 tiles, u64 *total) {
+    u64 carry = 0;
+    for (u64 s = 0; s < tiles; s += 1024) {
+        const u64 i = s + threadIdx.x;
+        const u64 v = i < tiles ? tileSum[i] : 0;
+        u64 tot;
+        const u64 ex = block_excl_scan<1024>(v, &tot);
+        if (i < tiles) tileSum[i] = carry + ex;
+        carry += tot;
+    }
+    if (threadIdx.x == 0) *total = carry;
+}
+
+__global__ __launch_bounds__(SCAN_WG) void k_scan_down(const u32 *__restrict__ c, u64 n, const u64 *__restrict__ tileSum,
+                                                      u64 *__restrict__ off) {
+    const u64 base = (u64)blockIdx.x * SCAN_TILE + (u64)threadIdx.x * SCAN_IPT;
+    u32 v[SCAN_IPT];
+    load_tile(c, n, base, v);
+    u64 s = 0;
+#pragma unroll
+    for (int j = 0; j < SCAN_IPT; j++) s += v[j];
+    u64 tot;
+    u64 run = tileSum[blockIdx.x] + block_excl_scan<SCAN_WG>(s, &tot);
+#pragma unroll
+    for (int j = 0; j < SCAN_IPT; j++) {
+        if (base + j < n) off[base + j] = run;
+        run += v[j];
+    }
+}
+
+// ---------------------------------------------------------------- probes
+template <int V>
+__global__ __launch_bounds__(wg_size<V>()) void k_probe_generate(const Pos *__restrict__ b, u64 n, u16 *__restrict__ moves,
+                                                                 u32 *__restrict__ cnt, Tables t) {
+    stage_tables<V>(t);
+    PG_GRID_LOOP(i, n) cnt[i] = generate_any<V>(b[i], t, moves + i * 256);
+}
+
+__global__ __launch_bounds__(256) void k_probe_make(const Pos *__restrict__ b, const u16 *__restrict__ mv, u64 n,
+                                                    Pos *__restrict__ out, Key128 *keys, Tables t) {
+    PG_GRID_LOOP(i, n) {
+        Pos p = b[i];
+        if (keys) {
+            Key128 h = keys[i];
+            make_any_hash(p, mv[i], h, t);
+            keys[i] = h;
+        } else {
+            make_any(p, mv[i], t);
+        }
+        out[i] = p;
+    }
+}
+
+template <int V>
+__global__ __launch_bounds__(wg_size<V>()) void k_probe_slider(int rook, const u8 *__restrict__ sq, const u64 *__restrict__ occ,
+                                                               u64 n, u64 *__restrict__ out, Tables t) {
+    stage_tables<V>(t);
+    PG_GRID_LOOP(i, n) out[i] = rook ? Sl<V>::rook(t, sq[i], occ[i]) : Sl<V>::bishop(t, sq[i], occ[i]);
+}
+
+}  // namespace pg

@@ -1,0 +1,585 @@
+// movegen.h -- legal move counting, generation and make-move for gfx950.
+//
+// Written for 64-lane wavefronts: the leaf count is branch-light so that the
+// lanes of a wave (sibling positions) stay converged.
+//  * One "target" mask folds the reference's separate out-of-check path
+//    (MoveGeneratorBitboard.h:2825-2994) into the normal one: not in check ->
+//    ~own, single check -> checker | squares between, double check -> 0.
+//    A pinned piece can never land on a check-resolving square, so the
+//    result equals the reference's countMovesOutOfCheck.
+//  * Pawns and knights are counted in bulk (one shift + popcount per
+//    direction).  Pinned pawns move only along the king line they sit on,
+//    which is applied as a per-direction mask instead of a per-pawn loop.
+//  * With Kogge-Stone sliders, moves are also counted in bulk: the rays of
+//    different sliders in one direction are disjoint (a ray stops at the
+//    first occupied square), so popcount(fill_d(all movers) & target) sums
+//    the per-piece counts.  Pinned sliders join the fills of the axis their
+//    pin lies on.  The magic variants count per piece.
+//  * Enemy attacks (with the moving king removed, so sliders see through
+//    it), pins and slider checkers follow the reference's method
+//    (findAttackedSquares :1254, findPinnedPieces :1213).
+// The same source compiles for the host so the library can list moves and
+// build workloads without a GPU; perft itself only runs on the device.
+#pragma once
+#include "pg_types.h"
+
+namespace pg {
+
+enum Slider { SL_KOGGE = 0, SL_PLAIN = 1, SL_FANCY = 2, SL_BYTE = 3, SL_COUNT = 4 };
+
+// Fixed-shift fancy magic entry: factor, base offset into the shared
+// 97264-entry table, and (byte-lookup variant) offset into the per-piece list
+// of distinct attack sets.  Same fields as the reference's FancyMagicEntry
+// (chess.h:343-359).
+struct FancyEntry {
+    u64 factor;
+    int base;
+    int offset;
+};
+static_assert(sizeof(FancyEntry) == 16, "fancy entry is 16 bytes");
+
+constexpr int ROOK_BITS = 12, BISHOP_BITS = 9;
+constexpr int FANCY_TABLE_SIZE = 97264, BYTE_ROOK_SIZE = 4900, BYTE_BISHOP_SIZE = 1428;
+constexpr int ZOB_WORDS = 782;
+
+// Table pointers (device or host memory; on the device the byte-lookup
+// variant repoints some of them at LDS copies).
+struct Tables {
+    const u64 *knight, *king;            // [64] empty-board attacks
+    const u64 *rookEmpty, *bishopEmpty;  // [64]
+    const u64 *between;                  // [64*64] squares strictly between
+    const u64 *line;                     // [64*64] full line through two squares
+    const u64 *rookMask, *bishopMask;    // [64] relevant occupancy (edges off)
+    const u64 *rookMagic, *bishopMagic;  // [64] plain magic factors
+    const u64 *rookPlain, *bishopPlain;  // [64*4096], [64*512]
+    const FancyEntry *rookFancy, *bishopFancy;  // [64]
+    const u64 *fancy;                    // [97264]
+    const u8 *byteIdx;                   // [97264]
+    const u64 *byteRook, *byteBishop;    // [4900], [1428]
+    const u64 *zobLo, *zobHi;            // [782] each
+};
+
+PG_HD int popc(u64 x) { return __builtin_popcountll(x); }
+PG_HD int ctz(u64 x) { return __builtin_ctzll(x); }
+PG_HD u64 bit(int s) { return 1ULL << s; }
+PG_HD bool several(u64 x) { return (x & (x - 1)) != 0; }
+
+// A board the generator accepts: one king per side, no white bit on an empty
+// square and an en-passant field of 0 (none) or a file 1..8.  (Inputs arriving over the C ABI are checked with this.)
+PG_HD bool pos_sane(const Pos &p) {
+    const u64 occ = pos_occ(p);
+    return popc(p.kings & p.white) == 1 && popc(p.kings & ~p.white) == 1 && (p.white & ~occ) == 0 &&
+           pos_ep(p) <= 8;
+}
+
+// ---------------------------------------------------------------- fills
+// Kogge-Stone occluded fill, returns the squares attacked in one direction
+// (first blocker included).  S = shift, M = wrap mask of the target file.
+template <int S, u64 M>
+PG_HD u64 fill_up(u64 g, u64 e) {
+    e &= M;
+    g |= (g << S) & e;
+    e &= e << S;
+    g |= (g << (2 * S)) & e;
+    e &= e << (2 * S);
+    g |= (g << (4 * S)) & e;
+    return (g << S) & M;
+}
+template <int S, u64 M>
+PG_HD u64 fill_dn(u64 g, u64 e) {
+    e &= M;
+    g |= (g >> S) & e;
+    e &= e >> S;
+    g |= (g >> (2 * S)) & e;
+    e &= e >> (2 * S);
+    g |= (g >> (4 * S)) & e;
+    return (g >> S) & M;
+}
+PG_HD u64 att_n(u64 g, u64 e) { return fill_up<8, ~0ULL>(g, e); }
+PG_HD u64 att_s(u64 g, u64 e) { return fill_dn<8, ~0ULL>(g, e); }
+PG_HD u64 att_e(u64 g, u64 e) { return fill_up<1, ~FILEA>(g, e); }
+PG_HD u64 att_w(u64 g, u64 e) { return fill_dn<1, ~FILEH>(g, e); }
+PG_HD u64 att_ne(u64 g, u64 e) { return fill_up<9, ~FILEA>(g, e); }
+PG_HD u64 att_nw(u64 g, u64 e) { return fill_up<7, ~FILEH>(g, e); }
+PG_HD u64 att_se(u64 g, u64 e) { return fill_dn<7, ~FILEA>(g, e); }
+PG_HD u64 att_sw(u64 g, u64 e) { return fill_dn<9, ~FILEH>(g, e); }
+PG_HD u64 ks_rook(u64 g, u64 e) { return att_n(g, e) | att_s(g, e) | att_e(g, e) | att_w(g, e); }
+PG_HD u64 ks_bishop(u64 g, u64 e) { return att_ne(g, e) | att_nw(g, e) | att_se(g, e) | att_sw(g, e); }
+
+PG_HD u64 king_att(u64 k) {
+    u64 a = ((k << 1) & ~FILEA) | ((k >> 1) & ~FILEH);
+    k |= a;
+    return a | (k << 8) | (k >> 8);
+}
+PG_HD u64 knight_att(u64 n) {
+    u64 l1 = (n >> 1) & 0x7f7f7f7f7f7f7f7fULL, l2 = (n >> 2) & 0x3f3f3f3f3f3f3f3fULL;
+    u64 r1 = (n << 1) & 0xfefefefefefefefeULL, r2 = (n << 2) & 0xfcfcfcfcfcfcfcfcULL;
+    u64 h1 = l1 | r1, h2 = l2 | r2;
+    return (h1 << 16) | (h1 >> 16) | (h2 << 8) | (h2 >> 8);
+}
+// sum over all knights of their moves landing in t
+PG_HD int knight_count(u64 n, u64 t) {
+    const u64 AB = FILEA | FILEB, GH = FILEG | FILEH;
+    return popc((n << 17) & ~FILEA & t) + popc((n << 15) & ~FILEH & t) + popc((n << 10) & ~AB & t) +
+           popc((n << 6) & ~GH & t) + popc((n >> 17) & ~FILEH & t) + popc((n >> 15) & ~FILEA & t) +
+           popc((n >> 10) & ~GH & t) + popc((n >> 6) & ~AB & t);
+}
+
+// pawn geometry per side
+template <int C> PG_HD u64 fwd(u64 x) { return C == WHITE ? x << 8 : x >> 8; }
+template <int C> PG_HD u64 fwd_west(u64 x) { return C == WHITE ? (x << 7) & ~FILEH : (x >> 9) & ~FILEH; }
+template <int C> PG_HD u64 fwd_east(u64 x) { return C == WHITE ? (x << 9) & ~FILEA : (x >> 7) & ~FILEA; }
+template <int C> PG_HD u64 pawn_att(u64 x) { return fwd_west<C>(x) | fwd_east<C>(x); }
+
+// lines through a square
+PG_HD u64 file_of(int s) { return FILEA << (s & 7); }
+PG_HD u64 rank_of(int s) { return RANK1 << (s & 56); }
+PG_HD u64 diag_of(int s) {  // a1-h8 direction
+    int d = (s & 7) - (s >> 3);
+    return d >= 0 ? DIAG_A1H8 >> (8 * d) : DIAG_A1H8 << (-8 * d);
+}
+PG_HD u64 anti_of(int s) {  // a8-h1 direction
+    int d = (s & 7) + (s >> 3) - 7;
+    return d >= 0 ? DIAG_A8H1 << (8 * d) : DIAG_A8H1 >> (-8 * d);
+}
+
+// ---------------------------------------------------------------- sliders
+template <int V> struct Sl;
+
+template <> struct Sl<SL_KOGGE> {
+    static constexpr bool kBulk = true;
+    PG_HD static u64 rook(const Tables &, int s, u64 occ) { return ks_rook(bit(s), ~occ); }
+    PG_HD static u64 bishop(const Tables &, int s, u64 occ) { return ks_bishop(bit(s), ~occ); }
+};
+
+// high 32 bits of the low 64-bit word of a*b, i.e. (a*b) >> 32 mod 2^32:
+// three 32-bit multiplies instead of a full 64x64 product.
+PG_HD u32 mul_hi32_lo64(u64 a, u64 b) {
+    u32 al = (u32)a, ah = (u32)(a >> 32), bl = (u32)b, bh = (u32)(b >> 32);
+    u32 hi = (u32)(((u64)al * bl) >> 32);
+    return hi + al * bh + ah * bl;
+}
+
+template <> struct Sl<SL_PLAIN> {
+    static constexpr bool kBulk = false;
+    PG_HD static u64 rook(const Tables &t, int s, u64 occ) {
+        u32 i = mul_hi32_lo64(t.rookMagic[s], occ & t.rookMask[s]) >> (32 - ROOK_BITS);
+        return t.rookPlain[(s << ROOK_BITS) + i];
+    }
+    PG_HD static u64 bishop(const Tables &t, int s, u64 occ) {
+        u32 i = mul_hi32_lo64(t.bishopMagic[s], occ & t.bishopMask[s]) >> (32 - BISHOP_BITS);
+        return t.bishopPlain[(s << BISHOP_BITS) + i];
+    }
+};
+
+template <> struct Sl<SL_FANCY> {
+    static constexpr bool kBulk = false;
+    PG_HD static u64 rook(const Tables &t, int s, u64 occ) {
+        FancyEntry f = t.rookFancy[s];
+        return t.fancy[f.base + (mul_hi32_lo64(f.factor, occ & t.rookMask[s]) >> (32 - ROOK_BITS))];
+    }
+    PG_HD static u64 bishop(const Tables &t, int s, u64 occ) {
+        FancyEntry f = t.bishopFancy[s];
+        return t.fancy[f.base + (mul_hi32_lo64(f.factor, occ & t.bishopMask[s]) >> (32 - BISHOP_BITS))];
+    }
+};
+
+template <> struct Sl<SL_BYTE> {
+    static constexpr bool kBulk = false;
+    PG_HD static u64 rook(const Tables &t, int s, u64 occ) {
+        FancyEntry f = t.rookFancy[s];
+        u32 i = mul_hi32_lo64(f.factor, occ & t.rookMask[s]) >> (32 - ROOK_BITS);
+        return t.byteRook[t.byteIdx[f.base + i] + f.offset];
+    }
+    PG_HD static u64 bishop(const Tables &t, int s, u64 occ) {
+        FancyEntry f = t.bishopFancy[s];
+        u32 i = mul_hi32_lo64(f.factor, occ & t.bishopMask[s]) >> (32 - BISHOP_BITS);
+        return t.byteBishop[t.byteIdx[f.base + i] + f.offset];
+    }
+};
+
+// ---------------------------------------------------------------- context
+struct MoveCtx {
+    u64 occ, mine, enemy, myPawns, myKing, eRooks, pinned, checkers, threatened, target;
+    int k;
+};
+
+template <int C, int V>
+PG_HD void prologue(const Pos &p, const Tables &t, MoveCtx &x) {
+    const u64 pawns = pos_pawns(p);
+    x.occ = p.kings | pawns | p.knights | p.bishopQueens | p.rookQueens;
+    const u64 black = x.occ & ~p.white;
+    x.mine = C == WHITE ? p.white : black;
+    x.enemy = C == WHITE ? black : p.white;
+    x.myPawns = pawns & x.mine;
+    x.myKing = p.kings & x.mine;
+    x.k = ctz(x.myKing) & 63;  // in range even for a board without a king
+    const u64 eP = pawns & x.enemy, eN = p.knights & x.enemy, eK = p.kings & x.enemy;
+    const u64 eB = p.bishopQueens & x.enemy, eR = p.rookQueens & x.enemy;
+    x.eRooks = eR;
+
+    // squares the enemy attacks, sliders looking through our king
+    u64 a = pawn_att<1 - C>(eP) | knight_att(eN) | king_att(eK);
+    if (Sl<V>::kBulk) {
+        const u64 e = ~x.occ | x.myKing;
+        a |= ks_bishop(eB, e) | ks_rook(eR, e);
+    } else {
+        const u64 o = x.occ ^ x.myKing;
+        for (u64 b = eB; b; b &= b - 1) a |= Sl<V>::bishop(t, ctz(b), o);
+        for (u64 r = eR; r; r &= r - 1) a |= Sl<V>::rook(t, ctz(r), o);
+    }
+    x.threatened = a;
+
+    // sliders on a king line: no blocker -> checker, one blocker -> pinned
+    u64 cand = (t.rookEmpty[x.k] & eR) | (t.bishopEmpty[x.k] & eB);
+    u64 pinned = 0, chk = 0;
+    for (; cand; cand &= cand - 1) {
+        const int s = ctz(cand);
+        const u64 b = t.between[(x.k << 6) | s] & x.occ;
+        if (b == 0) chk |= bit(s);
+        else if (!several(b)) pinned |= b;
+    }
+    chk |= (knight_att(x.myKing) & eN) | (pawn_att<C>(x.myKing) & eP);
+    x.pinned = pinned & x.mine;
+    x.checkers = chk;
+    u64 tgt = ~x.mine;
+    if (chk) tgt &= several(chk) ? 0ULL : (chk | t.between[(x.k << 6) | ctz(chk)]);
+    x.target = tgt;
+}
+
+// Bounded move writer: never stores past `cap`, but keeps counting so a
+// caller can detect a mismatch with the counted size.
+struct MoveSink {
+    u16 *out;
+    u32 cap, n;
+    PG_HD void add(u16 m) {
+        if (n < cap) out[n] = m;
+        n++;
+    }
+};
+
+// en-passant: number of legal captures (MoveGeneratorBitboard.h:3050-3085,
+// and :2891-2902 when in check); writes them when `out` is given.
+template <int C>
+PG_HD int ep_moves(const Pos &p, const Tables &t, const MoveCtx &x, MoveSink *out) {
+    const int f = pos_ep(p);
+    if (!f) return 0;
+    const u64 dst = C == WHITE ? bit(39 + f) : bit(15 + f);
+    const u64 cap = C == WHITE ? dst >> 8 : dst << 8;
+    u64 src = (((cap << 1) & ~FILEA) | ((cap >> 1) & ~FILEH)) & x.myPawns;
+    int n = 0;
+    const int to = ctz(dst);
+    if (x.checkers) {
+        if (x.checkers != cap) return 0;
+        src &= ~x.pinned;
+        for (; src; src &= src - 1) {
+            if (out) out->add((u16)(ctz(src) | (to << 6) | (MF_EP_CAPTURE << 12)));
+            n++;
+        }
+        return n;
+    }
+    for (; src; src &= src - 1) {
+        const int s = ctz(src);
+        bool ok;
+        if (x.pinned & bit(s)) {
+            ok = (dst & t.line[(s << 6) | x.k]) != 0;
+        } else {
+            const u64 e = ~x.occ | cap | bit(s);
+            ok = ((att_e(x.eRooks, e) | att_w(x.eRooks, e)) & x.myKing) == 0;
+        }
+        if (ok) {
+            if (out) out->add((u16)(s | (to << 6) | (MF_EP_CAPTURE << 12)));
+            n++;
+        }
+    }
+    return n;
+}
+
+template <int C>
+PG_HD int castle_count(const Pos &p, const MoveCtx &x) {
+    if (x.checkers) return 0;
+    const int r = C == WHITE ? pos_wcastle(p) : pos_bcastle(p);
+    const int sh = C == WHITE ? 0 : 56;
+    int n = 0;
+    if ((r & 1) && !((0x60ULL << sh) & (x.occ | x.threatened))) n++;
+    if ((r & 2) && !((0x0EULL << sh) & x.occ) && !((0x0CULL << sh) & x.threatened)) n++;
+    return n;
+}
+
+// ---------------------------------------------------------------- count
+template <int C, int V>
+PG_HD u32 count_moves(const Pos &p, const Tables &t) {
+    MoveCtx x;
+    prologue<C, V>(p, t, x);
+    const u64 tgt = x.target, empty = ~x.occ, np = ~x.pinned;
+    const u64 PROMO = RANK1 | RANK8;
+    int n = popc(king_att(x.myKing) & ~x.mine & ~x.threatened);
+    if (tgt == 0) return n;  // double check: king moves only
+
+    const u64 fk = file_of(x.k), rk = rank_of(x.k), dk = diag_of(x.k), ak = anti_of(x.k);
+
+    // pawns
+    const u64 p1 = fwd<C>(x.myPawns & (np | fk)) & empty;
+    const u64 p2 = fwd<C>(p1 & (C == WHITE ? RANK3 : RANK6)) & empty & tgt;
+    const u64 t1 = p1 & tgt;
+    n += popc(t1) + 3 * popc(t1 & PROMO) + popc(p2);
+    const u64 cw = fwd_west<C>(x.myPawns & (np | (C == WHITE ? ak : dk))) & x.enemy & tgt;
+    const u64 ce = fwd_east<C>(x.myPawns & (np | (C == WHITE ? dk : ak))) & x.enemy & tgt;
+    n += popc(cw) + 3 * popc(cw & PROMO) + popc(ce) + 3 * popc(ce & PROMO);
+
+    // knights (pinned knights never move)
+    n += knight_count(p.knights & x.mine & np, tgt);
+
+    // sliders
+    const u64 rq = p.rookQueens & x.mine, bq = p.bishopQueens & x.mine;
+    if (Sl<V>::kBulk) {
+        const u64 rf = rq & (np | fk), rr = rq & (np | rk), bd = bq & (np | dk), ba = bq & (np | ak);
+        n += popc(att_n(rf, empty) & tgt) + popc(att_s(rf, empty) & tgt) +
+             popc(att_e(rr, empty) & tgt) + popc(att_w(rr, empty) & tgt) +
+             popc(att_ne(bd, empty) & tgt) + popc(att_sw(bd, empty) & tgt) +
+             popc(att_nw(ba, empty) & tgt) + popc(att_se(ba, empty) & tgt);
+    } else {
+        for (u64 b = bq; b; b &= b - 1) {
+            const int s = ctz(b);
+            u64 m = Sl<V>::bishop(t, s, x.occ) & tgt;
+            if (x.pinned & bit(s)) m &= t.line[(s << 6) | x.k];
+            n += popc(m);
+        }
+        for (u64 r = rq; r; r &= r - 1) {
+            const int s = ctz(r);
+            u64 m = Sl<V>::rook(t, s, x.occ) & tgt;
+            if (x.pinned & bit(s)) m &= t.line[(s << 6) | x.k];
+            n += popc(m);
+        }
+    }
+    n += castle_count<C>(p, x);
+    n += ep_moves<C>(p, t, x, nullptr);
+    return (u32)n;
+}
+
+// ---------------------------------------------------------------- generate
+PG_HD u16 mk_move(int from, int to, int flags) { return (u16)(from | (to << 6) | (flags << 12)); }
+
+// Writes the legal moves of p to out (at most cap of them); returns the
+// number of legal moves (at most 218).
+// Pawn captures carry the capture flag; other captures do not, as on the
+// reference's device path (MoveGeneratorBitboard.h:2525-2529).
+template <int C, int V>
+PG_HD u32 generate_moves(const Pos &p, const Tables &t, u16 *out, u32 cap) {
+    MoveCtx x;
+    prologue<C, V>(p, t, x);
+    const u64 tgt = x.target, empty = ~x.occ, np = ~x.pinned;
+    MoveSink o{out, cap, 0};
+    for (u64 m = king_att(x.myKing) & ~x.mine & ~x.threatened; m; m &= m - 1) o.add(mk_move(x.k, ctz(m), MF_QUIET));
+    if (tgt == 0) return o.n;
+
+    const u64 fk = file_of(x.k), dk = diag_of(x.k), ak = anti_of(x.k);
+    const int up = C == WHITE ? 8 : -8;
+    const u64 PROMO = RANK1 | RANK8;
+    // pushes
+    const u64 p1 = fwd<C>(x.myPawns & (np | fk)) & empty;
+    const u64 p2 = fwd<C>(p1 & (C == WHITE ? RANK3 : RANK6)) & empty & tgt;
+    for (u64 m = p1 & tgt; m; m &= m - 1) {
+        const int to = ctz(m), from = to - up;
+        if (bit(to) & PROMO) {
+            o.add(mk_move(from, to, MF_PROMO | 0));
+            o.add(mk_move(from, to, MF_PROMO | 1));
+            o.add(mk_move(from, to, MF_PROMO | 3));
+            o.add(mk_move(from, to, MF_PROMO | 2));
+        } else {
+            o.add(mk_move(from, to, MF_QUIET));
+        }
+    }
+    for (u64 m = p2; m; m &= m - 1) { const int to = ctz(m); o.add(mk_move(to - 2 * up, to, MF_DOUBLE_PUSH)); }
+    // captures
+    const u64 cw = fwd_west<C>(x.myPawns & (np | (C == WHITE ? ak : dk))) & x.enemy & tgt;
+    const u64 ce = fwd_east<C>(x.myPawns & (np | (C == WHITE ? dk : ak))) & x.enemy & tgt;
+    for (int side = 0; side < 2; side++) {
+        const int d = side == 0 ? (C == WHITE ? 7 : -9) : (C == WHITE ? 9 : -7);
+        for (u64 m = side == 0 ? cw : ce; m; m &= m - 1) {
+            const int to = ctz(m), from = to - d;
+            if (bit(to) & PROMO) {
+                o.add(mk_move(from, to, MF_PROMO | MF_CAPTURE | 0));
+                o.add(mk_move(from, to, MF_PROMO | MF_CAPTURE | 1));
+                o.add(mk_move(from, to, MF_PROMO | MF_CAPTURE | 3));
+                o.add(mk_move(from, to, MF_PROMO | MF_CAPTURE | 2));
+            } else {
+                o.add(mk_move(from, to, MF_CAPTURE));
+            }
+        }
+    }
+    ep_moves<C>(p, t, x, &o);
+    // castling
+    if (!x.checkers) {
+        const int r = C == WHITE ? pos_wcastle(p) : pos_bcastle(p);
+        const int sh = C == WHITE ? 0 : 56;
+        if ((r & 1) && !((0x60ULL << sh) & (x.occ | x.threatened))) o.add(mk_move(4 + sh, 6 + sh, MF_KING_CASTLE));
+        if ((r & 2) && !((0x0EULL << sh) & x.occ) && !((0x0CULL << sh) & x.threatened)) o.add(mk_move(4 + sh, 2 + sh, MF_QUEEN_CASTLE));
+    }
+    // knights
+    for (u64 b = p.knights & x.mine & np; b; b &= b - 1) {
+        const int s = ctz(b);
+        for (u64 m = t.knight[s] & tgt; m; m &= m - 1) o.add(mk_move(s, ctz(m), MF_QUIET));
+    }
+    // sliders
+    for (u64 b = p.bishopQueens & x.mine; b; b &= b - 1) {
+        const int s = ctz(b);
+        u64 m = Sl<V>::bishop(t, s, x.occ) & tgt;
+        if (x.pinned & bit(s)) m &= t.line[(s << 6) | x.k];
+        for (; m; m &= m - 1) o.add(mk_move(s, ctz(m), MF_QUIET));
+    }
+    for (u64 r = p.rookQueens & x.mine; r; r &= r - 1) {
+        const int s = ctz(r);
+        u64 m = Sl<V>::rook(t, s, x.occ) & tgt;
+        if (x.pinned & bit(s)) m &= t.line[(s << 6) | x.k];
+        for (; m; m &= m - 1) o.add(mk_move(s, ctz(m), MF_QUIET));
+    }
+    return o.n;
+}
+
+// ---------------------------------------------------------------- make move
+// Zobrist word offsets in the 782-word key set (chess.h:326-333)
+PG_HD int zob_piece(int color, int piece /*1..6*/, int sq) { return color * 384 + (piece - 1) * 64 + sq; }
+constexpr int ZOB_CASTLE = 768, ZOB_EP = 772, ZOB_SIDE = 780;
+enum { P_PAWN = 1, P_KNIGHT, P_BISHOP, P_ROOK, P_QUEEN, P_KING };
+
+PG_HD void zob_xor(Key128 &h, const Tables &t, int w) {
+    h.lo ^= t.zobLo[w];
+    h.hi ^= t.zobHi[w];
+}
+PG_HD void zob_state(Key128 &h, const Tables &t, const Pos &p) {
+    const int w = pos_wcastle(p), b = pos_bcastle(p), e = pos_ep(p);
+    if (w & 1) zob_xor(h, t, ZOB_CASTLE + 0);
+    if (w & 2) zob_xor(h, t, ZOB_CASTLE + 1);
+    if (b & 1) zob_xor(h, t, ZOB_CASTLE + 2);
+    if (b & 2) zob_xor(h, t, ZOB_CASTLE + 3);
+    if (e) zob_xor(h, t, ZOB_EP + e - 1);
+}
+
+PG_HD int piece_on(const Pos &p, u64 b) {
+    if (p.kings & b) return P_KING;
+    if (p.knights & b) return P_KNIGHT;
+    if (pos_pawns(p) & b) return P_PAWN;
+    if (p.bishopQueens & p.rookQueens & b) return P_QUEEN;
+    if (p.bishopQueens & b) return P_BISHOP;
+    if (p.rookQueens & b) return P_ROOK;
+    return 0;
+}
+
+// Applies a legal move (MoveGeneratorBitboard.h:3575-3836): the en-passant
+// file is recorded only when an enemy pawn can take, and castling rights are
+// cleared by king moves, rook moves from a corner and captures on a corner.
+// With HASH the 128-bit key is updated incrementally.
+template <int C, bool HASH>
+PG_HD void make_move(Pos &p, u16 mv, Key128 *h, const Tables &t) {
+    const int from = mv & 63, to = (mv >> 6) & 63, fl = mv >> 12;
+    const u64 src = bit(from), dst = bit(to);
+    int piece = piece_on(p, src);
+    if (!piece) piece = P_PAWN;  // only for a move that is not legal here; keeps key indices in range
+    if (HASH) {
+        zob_xor(*h, t, zob_piece(C, piece, from));
+        const int cap = piece_on(p, dst);
+        if (cap) zob_xor(*h, t, zob_piece(1 - C, cap, to));
+        zob_xor(*h, t, ZOB_SIDE);
+        zob_state(*h, t, p);
+    }
+    const bool movedRook = piece == P_ROOK;
+    if (fl & MF_PROMO) {
+        const int pr = fl & 3;  // N, B, R, Q
+        piece = pr == 0 ? P_KNIGHT : pr == 1 ? P_BISHOP : pr == 2 ? P_ROOK : P_QUEEN;
+    }
+    if (HASH) zob_xor(*h, t, zob_piece(C, piece, to));
+    const u64 clr = ~(src | dst);
+    p.bishopQueens &= clr;
+    p.rookQueens &= clr;
+    p.kings &= clr;
+    p.knights &= clr;
+    p.pawns &= ~((src | dst) & RANKS2TO7);
+    if (piece == P_KING) p.kings |= dst;
+    if (piece == P_KNIGHT) p.knights |= dst;
+    if (piece == P_PAWN) p.pawns |= dst;
+    if (piece == P_BISHOP || piece == P_QUEEN) p.bishopQueens |= dst;
+    if (piece == P_ROOK || piece == P_QUEEN) p.rookQueens |= dst;
+    if (C == WHITE) p.white = (p.white ^ src) | dst;
+    else p.white &= ~dst;
+
+    if (fl == MF_EP_CAPTURE) {
+        const u64 cap = C == WHITE ? dst >> 8 : dst << 8;
+        p.pawns &= ~cap;
+        if (C == BLACK) p.white &= ~cap;
+        if (HASH) zob_xor(*h, t, zob_piece(1 - C, P_PAWN, ctz(cap)));
+    }
+    if (fl == MF_KING_CASTLE || fl == MF_QUEEN_CASTLE) {
+        const int sh = C == WHITE ? 0 : 56;
+        const int rf = (fl == MF_KING_CASTLE ? 7 : 0) + sh, rt = (fl == MF_KING_CASTLE ? 5 : 3) + sh;
+        p.rookQueens = (p.rookQueens ^ bit(rf)) | bit(rt);
+        if (C == WHITE) p.white = (p.white ^ bit(rf)) | bit(rt);
+        if (HASH) {
+            zob_xor(*h, t, zob_piece(C, P_ROOK, rf));
+            zob_xor(*h, t, zob_piece(C, P_ROOK, rt));
+        }
+    }
+
+    // game state: side, en passant, castling
+    u64 st = p.pawns & ~(0xFFULL | (1ULL << 63));
+    int wc = pos_wcastle(p), bc = pos_bcastle(p);
+    if (piece == P_KING) { if (C == WHITE) wc = 0; else bc = 0; }
+    // a capture on the enemy's corner (updateCastleFlag, :1311)
+    if (C == WHITE) {
+        if (dst & bit(63)) bc &= ~1;
+        else if (dst & bit(56)) bc &= ~2;
+    } else {
+        if (dst & bit(7)) wc &= ~1;
+        else if (dst & bit(0)) wc &= ~2;
+    }
+    if (movedRook && piece == P_ROOK) {
+        if (C == WHITE) {
+            if (src & bit(7)) wc &= ~1;
+            else if (src & bit(0)) wc &= ~2;
+        } else {
+            if (src & bit(63)) bc &= ~1;
+            else if (src & bit(56)) bc &= ~2;
+        }
+    }
+    int epf = 0;
+    if (fl == MF_DOUBLE_PUSH) {
+        const u64 ePawns = (C == WHITE) ? (p.pawns & RANKS2TO7 & ~p.white) : (p.pawns & RANKS2TO7 & p.white);
+        if ((((dst << 1) & ~FILEA) | ((dst >> 1) & ~FILEH)) & ePawns) epf = (from & 7) + 1;
+    }
+    p.pawns = st | (u64)wc | ((u64)bc << 2) | ((u64)epf << 4) | ((u64)(1 - C) << 63);
+    if (HASH) zob_state(*h, t, p);
+}
+
+// Full 128-bit key (MoveGeneratorBitboard.h:3914-3980); the side key is
+// present when white is to move.
+PG_HD Key128 zobrist128(const Pos &p, const Tables &t) {
+    Key128 h = {0, 0};
+    if (pos_side(p) == WHITE) zob_xor(h, t, ZOB_SIDE);
+    zob_state(h, t, p);
+    for (u64 all = pos_occ(p); all; all &= all - 1) {
+        const int s = ctz(all);
+        const int color = (p.white >> s) & 1 ? WHITE : BLACK;
+        zob_xor(h, t, zob_piece(color, piece_on(p, bit(s)), s));
+    }
+    return h;
+}
+
+// side dispatch helpers
+template <int V>
+PG_HD u32 count_any(const Pos &p, const Tables &t) {
+    return pos_side(p) ? count_moves<BLACK, V>(p, t) : count_moves<WHITE, V>(p, t);
+}
+template <int V>
+PG_HD u32 generate_any(const Pos &p, const Tables &t, u16 *out, u32 cap = 256) {
+    return pos_side(p) ? generate_moves<BLACK, V>(p, t, out, cap) : generate_moves<WHITE, V>(p, t, out, cap);
+}
+PG_HD void make_any(Pos &p, u16 mv, const Tables &t) {
+    if (pos_side(p)) make_move<BLACK, false>(p, mv, nullptr, t);
+    else make_move<WHITE, false>(p, mv, nullptr, t);
+}
+PG_HD void make_any_hash(Pos &p, u16 mv, Key128 &h, const Tables &t) {
+    if (pos_side(p)) make_move<BLACK, true>(p, mv, &h, t);
+    else make_move<WHITE, true>(p, mv, &h, t);
+}
+
+}  // namespace pg

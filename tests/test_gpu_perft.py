@@ -1,0 +1,58 @@
+"""GPU perft parity: breadth-first perft on the device against published CPW
+values, the reference's own CPU perft (golden fixtures) and the oracle."""
+import pytest
+
+from conftest import golden
+from perft_gpu_amd import workloads as W
+
+pytestmark = pytest.mark.gpu
+VARIANTS = [0, 1, 2, 3]
+
+
+@pytest.mark.parametrize("v", VARIANTS)
+def test_start_position_to_depth7(dev, v):
+    for d in range(0, 8):
+        print("start d%d variant %d" % (d, v), flush=True)
+        assert dev.perft(W.CPW["start"], d, v) == W.CPW_COUNTS["start"][d]
+
+
+@pytest.mark.parametrize("name", ["kiwipete", "pos3", "pos4", "pos4_mirror", "pos5_old", "max218"])
+def test_cpw_positions(dev, name):
+    item = golden("cpw.json")[name]
+    for v in VARIANTS:
+        for d, want in enumerate(item["perft"]):
+            if d > 5 and v != 0:
+                continue
+            print(name, "d%d variant %d" % (d, v), flush=True)
+            assert dev.perft(item["fen"], d, v) == want
+
+
+def test_random_positions_depth4(dev, pg):
+    for item in golden("random.json")[:128]:
+        p = pg.pos_from_words(item["words"])
+        for v in VARIANTS:
+            assert dev.perft(p, 4, v) == item["perft"][4], (item["fen"], v)
+
+
+@pytest.mark.parametrize("v", VARIANTS)
+def test_c2_random_positions_depth6(dev, pg, v):
+    c2 = golden("c2.json")
+    for item in c2["positions"]:
+        print("c2", item["fen"], "variant", v, flush=True)
+        assert dev.perft(pg.pos_from_words(item["words"]), 6, v) == item["perft"][6]
+
+
+def test_start_depth8(dev):
+    print("start d8", flush=True)
+    assert dev.perft(W.CPW["start"], 8, 0) == W.CPW_COUNTS["start"][8]
+    st = dev.last_stats()
+    assert st["leaves"] == W.CPW_COUNTS["start"][8]
+    assert st["nodes"][1:8] == W.CPW_COUNTS["start"][1:8]
+
+
+def test_small_arena_forces_splits_and_stays_exact(pg):
+    # 256 MiB cannot hold start-position ply 6 (119M boards), so levels split
+    with pg.Device(0, arena_bytes=256 << 20) as small:
+        print("split arena start d6", flush=True)
+        assert small.perft(W.CPW["start"], 6, 0) == W.CPW_COUNTS["start"][6]
+        assert small.last_stats()["splits"] > 0

@@ -1,0 +1,96 @@
+"""The oracle (oracle/perft_oracle.c) pinned before it is trusted:
+public CPW perft values, the reference's own compiled CPU code (golden
+fixtures written by tests/golden/make_golden.py from oracle/_ref), and
+invariants the mathematics fixes."""
+import pytest
+
+from conftest import golden
+from perft_gpu_amd import workloads as W
+
+
+@pytest.mark.parametrize("name", ["start", "kiwipete", "pos3", "pos4", "pos4_mirror", "max218"])
+def test_cpw_published_counts(orc, name):
+    # CPW values are independent of the reference; depths kept CPU-cheap
+    cap = {"start": 5, "kiwipete": 4, "pos3": 6, "pos4": 5, "pos4_mirror": 5, "max218": 1}[name]
+    p = orc.fen(W.CPW[name])
+    for d in range(1, cap + 1):
+        assert orc.perft(p, d) == W.CPW_COUNTS[name][d], (name, d)
+
+
+def test_cpw_reference_counts(orc):
+    # perft from the reference's own CPU path, incl. the old position 5 (no CPW table in-tree)
+    for name, item in golden("cpw.json").items():
+        p = orc.fen(item["fen"])
+        assert list(p) == item["words"], name
+        for d, want in enumerate(item["perft"]):
+            if want > 250_000_000:
+                continue
+            assert orc.perft(p, d) == want, (name, d)
+
+
+def test_start_depth6_multithreaded(orc):
+    p = orc.fen(W.CPW["start"])
+    assert orc.perft_mt(p, 6, 4) == 119060324
+
+
+def test_random_positions_match_reference(orc):
+    for item in golden("random.json"):
+        w = tuple(item["words"])
+        for d in range(1, 5):
+            assert orc.perft(w, d) == item["perft"][d], (item["fen"], d)
+
+
+def test_movegen_matches_reference(orc):
+    for item in golden("movegen.json"):
+        w = tuple(item["words"])
+        assert orc.count_moves(w) == item["count"]
+        assert sorted(orc.generate_moves(w)) == item["moves"]
+
+
+def test_generate_boards_agrees_with_moves(orc):
+    # generateBoards (CPU recursion) and generateMoves+makeMove (GPU path) give
+    # the same children up to the en-passant flag and the clock byte
+    mask = ~((0xF << 4) | (0x7F << 56)) & (2**64 - 1)
+    for item in golden("movegen.json")[:128]:
+        w = tuple(item["words"])
+        boards = sorted(tuple(x if i != 1 else x & mask for i, x in enumerate(b)) for b in orc.generate_boards(w))
+        made = []
+        for m in orc.generate_moves(w):
+            c, _ = orc.make_move(w, m)
+            made.append(tuple(x if i != 1 else x & mask for i, x in enumerate(c)))
+        assert boards == sorted(made)
+
+
+def test_zobrist_reference_keys(orc):
+    for item in golden("zobrist.json"):
+        w = tuple(item["words"])
+        assert list(orc.zobrist128(w)) == item["key"]
+        for ch in item["children"]:
+            child, key = orc.make_move(w, ch["move"], tuple(item["key"]))
+            assert list(child) == ch["child"]
+            assert list(key) == ch["key"]
+
+
+def test_zobrist_incremental_equals_full(orc):
+    # invariant: the incrementally updated key equals the key of the child
+    for item in golden("zobrist.json"):
+        w = tuple(item["words"])
+        k = orc.zobrist128(w)
+        for m in orc.generate_moves(w):
+            child, ck = orc.make_move(w, m, k)
+            assert ck == orc.zobrist128(child)
+
+
+def test_perft_is_sum_of_divide(orc):
+    p = orc.fen(W.CPW["kiwipete"])
+    div = orc.divide(p, 3)
+    assert len(div) == 48 and sum(div.values()) == 97862
+
+
+def test_planted_mistake_is_caught(orc):
+    # a plausible bug (dropping one legal move somewhere) must fail the pins:
+    # perft(n) over children must equal the parent's perft(n+1)
+    p = orc.fen(W.CPW["pos3"])
+    kids = orc.generate_boards(p)
+    assert sum(orc.perft(k, 3) for k in kids) == W.CPW_COUNTS["pos3"][4]
+    assert sum(orc.perft(k, 3) for k in kids[1:]) != W.CPW_COUNTS["pos3"][4]
