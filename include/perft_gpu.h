@@ -130,6 +130,11 @@ int pg_device_make_move(pg_ctx *ctx, const pg_pos48 *pos, const pg_move *moves, 
 int pg_device_slider_attacks(pg_ctx *ctx, int slider, int rook, const uint8_t *squares,
                              const uint64_t *occupancy, uint32_t n, uint64_t *out);
 
+/* Plain device-to-device copy of `bytes` (16-byte lanes), `iters` times;
+ * *ms gets the mean launch time.  No reference counterpart: it calibrates
+ * the HBM byte counters and the achievable copy bandwidth. */
+int pg_device_copy(pg_ctx *ctx, uint64_t bytes, uint32_t iters, double *ms);
+
 #ifdef __cplusplus
 }
 #endif

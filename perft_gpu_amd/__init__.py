@@ -114,6 +114,7 @@ def lib():
                                                ctypes.c_void_p, ctypes.c_void_p]),
         "pg_device_slider_attacks": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.c_void_p,
                                                     ctypes.c_void_p, ctypes.c_uint32, ctypes.c_void_p]),
+        "pg_device_copy": (ctypes.c_int, [ctypes.c_void_p, u64, ctypes.c_uint32, P(ctypes.c_double)]),
     }
     for name, (res, args) in sig.items():
         f = getattr(L, name)
@@ -289,6 +290,13 @@ class Device:
         _check(lib().pg_device_slider_attacks(self._h, slider, int(rook), _ptr(sq), _ptr(occ), len(sq), _ptr(out)),
                "pg_device_slider_attacks")
         return out
+
+
+    def copy_bandwidth(self, nbytes, iters=10):
+        """Mean time (ms) and GB/s (read + write) of a plain nbytes copy."""
+        ms = ctypes.c_double()
+        _check(lib().pg_device_copy(self._h, nbytes, iters, ctypes.byref(ms)), "pg_device_copy")
+        return ms.value, 2 * nbytes / (ms.value * 1e-3) / 1e9
 
 
 def init_gpu(device=0, **kw):

@@ -149,7 +149,6 @@ int Engine::bfs(int v, const Pos *boards, u64 n, const u32 *cnt, int r, int ply)
     int rc = off ? scan(c, n, off, &total) : PG_ENOMEM;
     if (rc == PG_ENOMEM) { top = mark; return split(v, boards, n, cnt, r, ply); }
     if (rc) return rc;
-    if (ply + 1 < 16) stats.nodes[ply + 1] += total;
     if (total == 0) { top = mark; return PG_OK; }
 
     Pos *children = nullptr;
@@ -165,6 +164,7 @@ int Engine::bfs(int v, const Pos *boards, u64 n, const u32 *cnt, int r, int ply)
         if (n == 1) { set_error("one board's children do not fit the arena (ply %d)", ply); return PG_ENOMEM; }
         return split(v, boards, n, cnt, r, ply);
     }
+    if (ply + 1 < 16) stats.nodes[ply + 1] += total;
     PG_VARIANT_CALL(v, launch_generate, *this, boards, n, off, mv, par);
     HIP_TRY(hipGetLastError());
     if (r == 2) {
@@ -448,6 +448,26 @@ int pg_device_slider_attacks(pg_ctx *c, int slider, int rook, const uint8_t *sq,
     PG_VARIANT_CALL(v, launch_probe_slider, *c, rook, (const u8 *)ds, (const u64 *)dO, n, (u64 *)dout);
     if (hipGetLastError() != hipSuccess) return PG_EHIP;
     PROBE_TRY(stage_out(c, out, dout, n * sizeof(u64)));
+    c->top = 0;
+    return PG_OK;
+}
+
+int pg_device_copy(pg_ctx *c, uint64_t bytes, uint32_t iters, double *ms_out) {
+    if (!c || bytes < 16 || iters == 0) return PG_EINVAL;
+    (void)hipSetDevice(c->device);
+    c->top = 0;
+    const u64 n = bytes / 16;
+    void *a = c->alloc(n * 16), *b = c->alloc(n * 16);
+    if (!a || !b) { c->top = 0; return PG_ENOMEM; }
+    if (hipMemsetAsync(a, 1, n * 16, c->stream) != hipSuccess) return PG_EHIP;
+    const u32 g = c->grid(n, 256);
+    if (hipEventRecord(c->ev0, c->stream) != hipSuccess) return PG_EHIP;
+    for (uint32_t i = 0; i < iters; i++) k_copy16<<<g, 256, 0, c->stream>>>((const uint4 *)a, (uint4 *)b, n);
+    if (hipEventRecord(c->ev1, c->stream) != hipSuccess) return PG_EHIP;
+    if (hipEventSynchronize(c->ev1) != hipSuccess) return PG_EHIP;
+    float ms = 0;
+    (void)hipEventElapsedTime(&ms, c->ev0, c->ev1);
+    if (ms_out) *ms_out = ms / iters;
     c->top = 0;
     return PG_OK;
 }

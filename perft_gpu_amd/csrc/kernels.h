@@ -232,6 +232,12 @@ __global__ __launch_bounds__(SCAN_WG) void k_scan_down(const u32 *__restrict__ c
 }
 
 // ---------------------------------------------------------------- probes
+// plain 16-byte copy of a known size: the reference point that scales the
+// HBM byte counters of the other kernels
+__global__ __launch_bounds__(256) void k_copy16(const uint4 *__restrict__ src, uint4 *__restrict__ dst, u64 n) {
+    PG_GRID_LOOP(i, n) dst[i] = src[i];
+}
+
 template <int V>
 __global__ __launch_bounds__(wg_size<V>()) void k_probe_generate(const Pos *__restrict__ b, u64 n, u16 *__restrict__ moves,
                                                                  u32 *__restrict__ cnt, Tables t) {

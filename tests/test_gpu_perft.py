@@ -51,8 +51,12 @@ def test_start_depth8(dev):
 
 
 def test_small_arena_forces_splits_and_stays_exact(pg):
-    # 256 MiB cannot hold start-position ply 6 (119M boards), so levels split
-    with pg.Device(0, arena_bytes=256 << 20) as small:
-        print("split arena start d6", flush=True)
-        assert small.perft(W.CPW["start"], 6, 0) == W.CPW_COUNTS["start"][6]
-        assert small.last_stats()["splits"] > 0
+    # perft(7) needs ~250 MB for ply 5 and ~700 MB for the ply-6 items; a
+    # 64 MiB arena forces both levels to split
+    with pg.Device(0, arena_bytes=64 << 20) as small:
+        for v in (0, 3):
+            print("split arena start d7 variant", v, flush=True)
+            assert small.perft(W.CPW["start"], 7, v) == W.CPW_COUNTS["start"][7]
+            st = small.last_stats()
+            assert st["splits"] > 0 and st["peak_arena"] <= 64 << 20
+            assert st["nodes"][1:7] == W.CPW_COUNTS["start"][1:7]
