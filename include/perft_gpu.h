@@ -61,6 +61,7 @@ typedef struct pg_config {
     int block_size;        /* work-group size for the BFS kernels, 0 = 256 */
     int blocks_per_cu;     /* resident work-groups per CU for grid-stride kernels, 0 = 8 */
     int timing;            /* 1: time each leaf-kernel launch with HIP events */
+    int leaf_mode;         /* 0: last two plies fused in LDS (default), 1: materialized item list */
 } pg_config;
 
 typedef struct pg_stats {

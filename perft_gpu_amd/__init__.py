@@ -53,7 +53,7 @@ class Key128(ctypes.Structure):
 
 class Config(ctypes.Structure):
     _fields_ = [("arena_bytes", u64), ("slider", ctypes.c_int), ("block_size", ctypes.c_int),
-                ("blocks_per_cu", ctypes.c_int), ("timing", ctypes.c_int)]
+                ("blocks_per_cu", ctypes.c_int), ("timing", ctypes.c_int), ("leaf_mode", ctypes.c_int)]
 
 
 class Stats(ctypes.Structure):
@@ -214,9 +214,9 @@ def pos_from_words(w):
 class Device:
     """One GPU context (pg_ctx): tables uploaded, tree arena reserved."""
 
-    def __init__(self, device=0, arena_bytes=0, slider=SLIDER_AUTO, timing=False, blocks_per_cu=0):
+    def __init__(self, device=0, arena_bytes=0, slider=SLIDER_AUTO, timing=False, blocks_per_cu=0, leaf_mode=0):
         L = lib()
-        cfg = Config(arena_bytes, slider, 0, blocks_per_cu, 1 if timing else 0)
+        cfg = Config(arena_bytes, slider, 0, blocks_per_cu, 1 if timing else 0, leaf_mode)
         h = ctypes.c_void_p()
         _check(L.pg_init(device, ctypes.byref(cfg), ctypes.byref(h)), "pg_init")
         self._h = h

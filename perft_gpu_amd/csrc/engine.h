@@ -22,6 +22,8 @@ struct Engine {
     size_t arenaBytes = 0, top = 0, peak = 0;
     u8 *dSmall = nullptr;    // accumulator and error flag
     u64 *dAcc = nullptr;
+    u64 *dItems = nullptr;   // items consumed by fused leaf launches
+    int leafMode = 0;        // 0 fused last two plies, 1 materialized item list
     u32 *dErr = nullptr;
     u64 *hostBuf = nullptr;  // pinned
     pg_stats stats{};
@@ -31,6 +33,7 @@ struct Engine {
     int read_u64(const u64 *d, u64 *h);
     int scan(const u32 *cnt, u64 n, u64 *off, u64 *total);
     int leaf(int v, const Pos *b, u64 n, const u32 *par, const u16 *mv, u64 m);
+    int leaf_fused(int v, const Pos *b, const u32 *cnt, u64 n);
     int bfs(int v, const Pos *boards, u64 n, const u32 *cnt, int r, int ply);
     int split(int v, const Pos *boards, u64 n, const u32 *cnt, int r, int ply);
     int perft(const Pos &root, u32 depth, int v, u64 *out);

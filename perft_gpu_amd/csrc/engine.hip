@@ -126,6 +126,30 @@ int Engine::leaf(int v, const Pos *b, u64 n, const u32 *par, const u16 *mv, u64 
     return PG_OK;
 }
 
+template <int V> static void launch_leaf_fused(Engine &c, const Pos *b, const u32 *cnt, u64 n) {
+    if constexpr (V == SL_BYTE) {
+        (void)c; (void)b; (void)cnt; (void)n;
+    } else {
+        k_leaf_fused<V, false><<<c.grid(n, LEAF_WG), LEAF_WG, 0, c.stream>>>(b, cnt, n, c.dAcc, c.dItems, nullptr, c.dErr, c.dt);
+    }
+}
+
+int Engine::leaf_fused(int v, const Pos *b, const u32 *cnt, u64 n) {
+    if (timing) HIP_TRY(hipEventRecord(ev0, stream));
+    PG_VARIANT_CALL(v, launch_leaf_fused, *this, b, cnt, n);
+    HIP_TRY(hipGetLastError());
+    if (timing) {
+        HIP_TRY(hipEventRecord(ev1, stream));
+        HIP_TRY(hipEventSynchronize(ev1));
+        float ms = 0;
+        HIP_TRY(hipEventElapsedTime(&ms, ev0, ev1));
+        stats.leaf_kernel_ms += ms;
+    }
+    stats.leaf_launches++;
+    stats.leaf_parents += n;
+    return PG_OK;
+}
+
 // Sum of perft(r) over n boards that sit at `ply` below the root.  cnt, when
 // given, already holds each board's move count.
 int Engine::bfs(int v, const Pos *boards, u64 n, const u32 *cnt, int r, int ply) {
@@ -143,6 +167,12 @@ int Engine::bfs(int v, const Pos *boards, u64 n, const u32 *cnt, int r, int ply)
         c = (u32 *)alloc(n * sizeof(u32));
         if (!c) { top = mark; return split(v, boards, n, cnt, r, ply); }
         PG_VARIANT_CALL(v, launch_count, *this, boards, n, c);
+    }
+    if (r == 2 && leafMode == 0 && v != SL_BYTE) {
+        // last two plies in one kernel; the items never reach HBM
+        const int rc = leaf_fused(v, boards, c, n);
+        top = mark;
+        return rc;
     }
     u64 *off = (u64 *)alloc((n + 1) * sizeof(u64));
     u64 total = 0;
@@ -201,7 +231,7 @@ int Engine::perft(const Pos &root, u32 depth, int v, u64 *out) {
     Pos *d = (Pos *)alloc(sizeof(Pos));
     if (!d) return PG_ENOMEM;
     HIP_TRY(hipMemcpyAsync(d, &root, sizeof(Pos), hipMemcpyHostToDevice, stream));
-    HIP_TRY(hipMemsetAsync(dAcc, 0, sizeof(u64), stream));
+    HIP_TRY(hipMemsetAsync(dAcc, 0, 2 * sizeof(u64), stream));
     HIP_TRY(hipMemsetAsync(dErr, 0, sizeof(u32), stream));
     int rc = bfs(v, d, 1, nullptr, (int)depth, 0);
     if (rc) return rc;
@@ -213,6 +243,13 @@ int Engine::perft(const Pos &root, u32 depth, int v, u64 *out) {
     if (err) {
         set_error("move generator count/generate mismatch (flag %u)", err);
         return PG_EINVAL;
+    }
+    if (leafMode == 0) {
+        u64 items = 0;
+        rc = read_u64(dItems, &items);
+        if (rc) return rc;
+        stats.leaf_items += items;
+        if (depth >= 2 && depth - 1 < 16) stats.nodes[depth - 1] += items;  // fused launches all sit at ply depth-2
     }
     stats.leaves = res;
     stats.peak_arena = peak;
@@ -261,7 +298,7 @@ int pg_init(int device, const pg_config *cfg, pg_ctx **out) {
     }
     pg_ctx *c = new pg_ctx();
     c->device = device;
-    pg_config def = {0, PG_SLIDER_AUTO, 0, 0, 0};
+    pg_config def = {0, PG_SLIDER_AUTO, 0, 0, 0, 0};
     c->cfg = cfg ? *cfg : def;
     c->slider = (c->cfg.slider >= 0 && c->cfg.slider < SL_COUNT) ? c->cfg.slider : SL_KOGGE;
     c->bpc = c->cfg.blocks_per_cu > 0 ? c->cfg.blocks_per_cu : 8;
@@ -282,6 +319,8 @@ int pg_init(int device, const pg_config *cfg, pg_ctx **out) {
     c->dt = tables_at(c->dTables);
     if (hipMalloc(&c->dSmall, 256) != hipSuccess) return fail(PG_ENOMEM);
     c->dAcc = (u64 *)c->dSmall;
+    c->dItems = (u64 *)(c->dSmall + 8);
+    c->leafMode = c->cfg.leaf_mode;
     c->dErr = (u32 *)(c->dSmall + 64);
     if (hipHostMalloc(&c->hostBuf, 64) != hipSuccess) return fail(PG_ENOMEM);
     size_t freeB = 0, totalB = 0;

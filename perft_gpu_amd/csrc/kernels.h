@@ -140,11 +140,7 @@ __global__ __launch_bounds__(wg_size<V>()) void k_leaf(const Pos *__restrict__ p
     if ((threadIdx.x & 63) == 0 && s) atomicAdd(acc, s);
 }
 
-// ---------------------------------------------------------------- scan
-// Exclusive scan of u32 counts into u64 offsets; off[n] receives the total.
-// Reduce-then-scan over 2048-item tiles: k_scan_reduce (tile sums),
-// k_scan_spine (one work-group scans the tile sums), k_scan_down (each tile
-// rescans itself from its base).
+// ---------------------------------------------------------------- scan helpers
 constexpr int SCAN_WG = 256, SCAN_IPT = 8, SCAN_TILE = SCAN_WG * SCAN_IPT;
 
 __device__ __forceinline__ u64 wave_incl_scan(u64 v) {
@@ -177,6 +173,74 @@ __device__ __forceinline__ u64 block_excl_scan(u64 v, u64 *total) {
     return base + incl - v;
 }
 
+// ---------------------------------------------------------------- fused leaf
+// The last two plies without touching HBM for the (parent, move) items: a
+// work-group takes 256 boards of ply D-2 and their move counts, scans the
+// counts in LDS, generates the moves into an LDS window of LEAF_CAP slots
+// (boards that do not fit wait for the next window) and counts the legal
+// moves of every child.  PER = also write each board's perft(2).
+constexpr int LEAF_WG = 256, LEAF_CAP = 6144;
+
+template <int V, bool PER>
+__global__ __launch_bounds__(LEAF_WG) void k_leaf_fused(const Pos *__restrict__ boards, const u32 *__restrict__ cnt, u64 n,
+                                                        u64 *acc, u64 *items, u64 *__restrict__ per, u32 *err, Tables t) {
+    static_assert(V != SL_BYTE, "the byte-lookup tables leave no LDS for the move window");
+    __shared__ Pos sb[LEAF_WG];
+    __shared__ u16 smv[LEAF_CAP];
+    __shared__ u8 sown[LEAF_CAP];
+    __shared__ u32 sper[PER ? LEAF_WG : 1];
+    __shared__ u32 snext;
+    const int tid = threadIdx.x;
+    u64 s = 0;
+    for (u64 tile = blockIdx.x; tile * LEAF_WG < n; tile += gridDim.x) {
+        const u64 i = tile * LEAF_WG + tid;
+        const bool live = i < n;
+        const u32 c = live ? cnt[i] : 0;
+        if (live) sb[tid] = boards[i];
+        if (PER) sper[tid] = 0;
+        u64 total64;
+        const u32 pos = (u32)block_excl_scan<LEAF_WG>(c, &total64);
+        const u32 total = (u32)total64;
+        if (tid == 0 && items) atomicAdd(items, total64);
+        u32 start = 0;
+        while (start < total) {
+            const u32 end = start + LEAF_CAP;
+            if (tid == 0) snext = total;
+            __syncthreads();
+            const bool here = c && pos >= start && pos + c <= end;
+            if (here) {
+                const u32 g = generate_any<V>(sb[tid], t, smv + (pos - start), c);
+                if (g != c) *err = 2;
+                for (u32 j = 0; j < c; j++) sown[pos - start + j] = (u8)tid;
+            } else if (c && pos >= start) {
+                atomicMin(&snext, pos);
+            }
+            __syncthreads();
+            const u32 wend = snext;
+            for (u32 j = start + tid; j < wend; j += LEAF_WG) {
+                const u32 k = j - start;
+                const int b = sown[k];
+                const u32 m = count_after<V>(sb[b], smv[k], t);
+                if (PER) atomicAdd(&sper[b], m);
+                else s += m;
+            }
+            __syncthreads();
+            start = wend;
+        }
+        if (PER && live) per[i] = sper[tid];
+        __syncthreads();
+    }
+    if (!PER) {
+        s = wave_sum(s);
+        if ((tid & 63) == 0 && s) atomicAdd(acc, s);
+    }
+}
+
+// ---------------------------------------------------------------- scan
+// Exclusive scan of u32 counts into u64 offsets; off[n] receives the total.
+// Reduce-then-scan over 2048-item tiles: k_scan_reduce (tile sums),
+// k_scan_spine (one work-group scans the tile sums), k_scan_down (each tile
+// rescans itself from its base).
 __device__ __forceinline__ void load_tile(const u32 *c, u64 n, u64 base, u32 v[SCAN_IPT]) {
     if (base + SCAN_IPT <= n && (reinterpret_cast<uintptr_t>(c + base) & 15) == 0) {
         const uint4 a = *reinterpret_cast<const uint4 *>(c + base);
