@@ -131,6 +131,32 @@ int pg_device_make_move(pg_ctx *ctx, const pg_pos48 *pos, const pg_move *moves, 
 int pg_device_slider_attacks(pg_ctx *ctx, int slider, int rook, const uint8_t *squares,
                              const uint64_t *occupancy, uint32_t n, uint64_t *out);
 
+/* ------------------------------------------------- transposition table */
+
+typedef struct pg_tt pg_tt;
+
+/* setupHashTables128b / allocAndClearMem (launcher.h:155-320): one HBM
+ * allocation of at most `bytes`, split into one table per remaining depth
+ * (16-byte entries for depths 2-4, 32-byte entries above), zeroed. */
+int pg_tt_create(pg_ctx *ctx, uint64_t bytes, pg_tt **tt);
+int pg_tt_clear(pg_tt *tt);
+/* freeHashTables (launcher.h:322) */
+void pg_tt_destroy(pg_tt *tt);
+/* allocated bytes and log2(slots) per remaining depth (-1: no table) */
+int pg_tt_info(pg_tt *tt, uint64_t *bytes, int32_t *bits_per_depth /* [16] */);
+/* PRINT_HASH_STATS counters (perft_bb.h:24-32), per remaining depth:
+ * probes, hits, stores, in-level duplicates -> out[16][4] */
+int pg_tt_counters(pg_tt *tt, uint64_t *out);
+/* perftLauncher, TT branch, driving perft_bb_gpu_simple_hash
+ * (launcher.h:1457-1466, perft_bb.h:1242-1589) for one position. */
+int pg_perft_tt(pg_ctx *ctx, pg_tt *tt, const pg_pos48 *pos, uint32_t depth, int slider, uint64_t *out);
+/* Test hook (no reference counterpart): overwrite every entry with seeded
+ * pseudo-random words; lookups must then miss, never return garbage. */
+int pg_tt_poison(pg_tt *tt, uint64_t seed);
+/* FIND_DUPLICATES_IN_BFS (switches.h:52): 1 (default) collapses identical
+ * positions within a level before expanding them. */
+int pg_set_dedup(pg_ctx *ctx, int on);
+
 /* Plain device-to-device copy of `bytes` (16-byte lanes), `iters` times;
  * *ms gets the mean launch time.  No reference counterpart: it calibrates
  * the HBM byte counters and the achievable copy bandwidth. */

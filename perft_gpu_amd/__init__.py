@@ -115,6 +115,15 @@ def lib():
         "pg_device_slider_attacks": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.c_void_p,
                                                     ctypes.c_void_p, ctypes.c_uint32, ctypes.c_void_p]),
         "pg_device_copy": (ctypes.c_int, [ctypes.c_void_p, u64, ctypes.c_uint32, P(ctypes.c_double)]),
+        "pg_tt_create": (ctypes.c_int, [ctypes.c_void_p, u64, P(ctypes.c_void_p)]),
+        "pg_tt_clear": (ctypes.c_int, [ctypes.c_void_p]),
+        "pg_tt_destroy": (None, [ctypes.c_void_p]),
+        "pg_tt_info": (ctypes.c_int, [ctypes.c_void_p, P(u64), ctypes.c_void_p]),
+        "pg_tt_counters": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p]),
+        "pg_tt_poison": (ctypes.c_int, [ctypes.c_void_p, u64]),
+        "pg_perft_tt": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, P(Pos48), ctypes.c_uint32, ctypes.c_int,
+                                       P(u64)]),
+        "pg_set_dedup": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int]),
     }
     for name, (res, args) in sig.items():
         f = getattr(L, name)
@@ -245,6 +254,9 @@ class Device:
         _check(lib().pg_get_config(self._h, ctypes.byref(c)), "pg_get_config")
         return {n: getattr(c, n) for n, _ in c._fields_}
 
+    def set_dedup(self, on):
+        _check(lib().pg_set_dedup(self._h, 1 if on else 0), "pg_set_dedup")
+
     def set_slider(self, slider):
         _check(lib().pg_set_slider(self._h, slider), "pg_set_slider")
 
@@ -297,6 +309,53 @@ class Device:
         ms = ctypes.c_double()
         _check(lib().pg_device_copy(self._h, nbytes, iters, ctypes.byref(ms)), "pg_device_copy")
         return ms.value, 2 * nbytes / (ms.value * 1e-3) / 1e9
+
+
+class TranspositionTable:
+    """Device TT (pg_tt): setupHashTables128b (launcher.h:293), zeroed."""
+
+    def __init__(self, dev, nbytes):
+        h = ctypes.c_void_p()
+        _check(lib().pg_tt_create(dev._h, nbytes, ctypes.byref(h)), "pg_tt_create")
+        self._h = h
+        self.dev = dev
+
+    def close(self):
+        if self._h:
+            lib().pg_tt_destroy(self._h)
+            self._h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def clear(self):
+        _check(lib().pg_tt_clear(self._h), "pg_tt_clear")
+
+    def poison(self, seed=1):
+        _check(lib().pg_tt_poison(self._h, seed), "pg_tt_poison")
+
+    def info(self):
+        b = u64()
+        bits = np.zeros(16, dtype=np.int32)
+        _check(lib().pg_tt_info(self._h, ctypes.byref(b), _ptr(bits)), "pg_tt_info")
+        return int(b.value), [int(x) for x in bits]
+
+    def counters(self):
+        """per remaining depth: (probes, hits, stores, duplicates)"""
+        c = np.zeros((16, 4), dtype=np.uint64)
+        _check(lib().pg_tt_counters(self._h, _ptr(c)), "pg_tt_counters")
+        return c
+
+    def perft(self, pos, depth, slider=SLIDER_AUTO):
+        if isinstance(pos, str):
+            pos = read_fen_string(pos)
+        out = u64()
+        _check(lib().pg_perft_tt(self.dev._h, self._h, ctypes.byref(pos), depth, slider, ctypes.byref(out)),
+               "pg_perft_tt")
+        return int(out.value)
 
 
 def init_gpu(device=0, **kw):

@@ -3,10 +3,31 @@
 #include <hip/hip_runtime.h>
 #include "../../include/perft_gpu.h"
 #include "tables.h"
+#include "tt.h"
 
 namespace pg {
 
 void set_error(const char *fmt, ...);
+
+#define HIP_TRY(expr)                                                   \
+    do {                                                                \
+        hipError_t e_ = (expr);                                         \
+        if (e_ != hipSuccess) {                                         \
+            set_error("HIP error %s at line %d", hipGetErrorString(e_), __LINE__); \
+            return PG_EHIP;                                             \
+        }                                                               \
+    } while (0)
+
+#define PG_VARIANT_CALL(v, fn, ...)                  \
+    switch (v) {                                     \
+    case SL_KOGGE: fn<SL_KOGGE>(__VA_ARGS__); break; \
+    case SL_PLAIN: fn<SL_PLAIN>(__VA_ARGS__); break; \
+    case SL_FANCY: fn<SL_FANCY>(__VA_ARGS__); break; \
+    default: fn<SL_BYTE>(__VA_ARGS__); break;        \
+    }
+
+
+struct TTable;
 
 struct Engine {
     int device = 0, cus = 0, bpc = 8, slider = SL_KOGGE;
@@ -37,6 +58,22 @@ struct Engine {
     int bfs(int v, const Pos *boards, u64 n, const u32 *cnt, int r, int ply);
     int split(int v, const Pos *boards, u64 n, const u32 *cnt, int r, int ply);
     int perft(const Pos &root, u32 depth, int v, u64 *out);
+
+    // transposition-table perft
+    bool dedup = true;
+    int tt_bfs(int v, TTable &tt, Pos *B, Key128 *K, u64 n, u32 *cnt, int r, u64 *P, int ply);
+    int tt_split(int v, TTable &tt, Pos *B, Key128 *K, u64 n, u32 *cnt, int r, u64 *P, int ply);
+    int perft_tt(TTable &tt, const Pos &root, u32 depth, int v, u64 *out);
+};
+
+// Device transposition table (the object behind pg_tt).
+struct TTable {
+    Engine *eng = nullptr;
+    u8 *mem = nullptr;
+    size_t bytes = 0;
+    TTView view{};
+    int bits[TT_MAX_DEPTH] = {0};
+    TTCounters *dCtr = nullptr;
 };
 
 }  // namespace pg

@@ -30,15 +30,6 @@ void set_error(const char *fmt, ...) {
     va_end(ap);
 }
 
-#define HIP_TRY(expr)                                                   \
-    do {                                                                \
-        hipError_t e_ = (expr);                                         \
-        if (e_ != hipSuccess) {                                         \
-            set_error("HIP error %s at line %d", hipGetErrorString(e_), __LINE__); \
-            return PG_EHIP;                                             \
-        }                                                               \
-    } while (0)
-
 static double now_ms() {
     return std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now().time_since_epoch()).count();
 }
@@ -86,14 +77,6 @@ template <int V> static void launch_leaf(Engine &c, const Pos *b, const u32 *par
     constexpr int W = wg_size<V>();
     k_leaf<V><<<c.grid(m, W), W, 0, c.stream>>>(b, par, mv, m, acc, c.dt);
 }
-
-#define PG_VARIANT_CALL(v, fn, ...)                  \
-    switch (v) {                                     \
-    case SL_KOGGE: fn<SL_KOGGE>(__VA_ARGS__); break; \
-    case SL_PLAIN: fn<SL_PLAIN>(__VA_ARGS__); break; \
-    case SL_FANCY: fn<SL_FANCY>(__VA_ARGS__); break; \
-    default: fn<SL_BYTE>(__VA_ARGS__); break;        \
-    }
 
 int Engine::scan(const u32 *cnt, u64 n, u64 *off, u64 *total) {
     const u64 tiles = (n + SCAN_TILE - 1) / SCAN_TILE;

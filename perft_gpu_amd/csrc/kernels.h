@@ -94,7 +94,7 @@ __global__ __launch_bounds__(wg_size<V>()) void k_count_sum(const Pos *__restric
     if ((threadIdx.x & 63) == 0 && s) atomicAdd(acc, s);
 }
 
-__global__ __launch_bounds__(256) void k_sum_u32(const u32 *__restrict__ c, u64 n, u64 *acc) {
+static __global__ __launch_bounds__(256) void k_sum_u32(const u32 *__restrict__ c, u64 n, u64 *acc) {
     u64 s = 0;
     PG_GRID_LOOP(i, n) s += c[i];
     s = wave_sum(s);
@@ -111,6 +111,7 @@ __global__ __launch_bounds__(wg_size<V>()) void k_generate(const Pos *__restrict
     PG_GRID_LOOP(i, n) {
         const u64 o = off[i];
         const u32 room = (u32)(off[i + 1] - o);
+        if (room == 0) continue;  // no moves, or a board resolved earlier
         const u32 c = generate_any<V>(b[i], t, moves + o, room);
         if (c != room) *err = 1;
         for (u32 j = 0; j < room; j++) parent[o + j] = (u32)i;
@@ -227,7 +228,7 @@ __global__ __launch_bounds__(LEAF_WG) void k_leaf_fused(const Pos *__restrict__ 
             __syncthreads();
             start = wend;
         }
-        if (PER && live) per[i] = sper[tid];
+        if (PER && live && pos_valid(sb[tid])) per[i] = sper[tid];
         __syncthreads();
     }
     if (!PER) {
@@ -253,7 +254,7 @@ __device__ __forceinline__ void load_tile(const u32 *c, u64 n, u64 base, u32 v[S
     }
 }
 
-__global__ __launch_bounds__(SCAN_WG) void k_scan_reduce(const u32 *__restrict__ c, u64 n, u64 *__restrict__ tileSum) {
+static __global__ __launch_bounds__(SCAN_WG) void k_scan_reduce(const u32 *__restrict__ c, u64 n, u64 *__restrict__ tileSum) {
     const u64 base = (u64)blockIdx.x * SCAN_TILE + (u64)threadIdx.x * SCAN_IPT;
     u32 v[SCAN_IPT];
     load_tile(c, n, base, v);
@@ -265,7 +266,7 @@ __global__ __launch_bounds__(SCAN_WG) void k_scan_reduce(const u32 *__restrict__
     if (threadIdx.x == 0) tileSum[blockIdx.x] = tot;
 }
 
-__global__ __launch_bounds__(1024) void k_scan_spine(u64 *tileSum, u64 tiles, u64 *total) {
+static __global__ __launch_bounds__(1024) void k_scan_spine(u64 *tileSum, u64 tiles, u64 *total) {
     u64 carry = 0;
     for (u64 s = 0; s < tiles; s += 1024) {
         const u64 i = s + threadIdx.x;
@@ -278,7 +279,7 @@ __global__ __launch_bounds__(1024) void k_scan_spine(u64 *tileSum, u64 tiles, u6
     if (threadIdx.x == 0) *total = carry;
 }
 
-__global__ __launch_bounds__(SCAN_WG) void k_scan_down(const u32 *__restrict__ c, u64 n, const u64 *__restrict__ tileSum,
+static __global__ __launch_bounds__(SCAN_WG) void k_scan_down(const u32 *__restrict__ c, u64 n, const u64 *__restrict__ tileSum,
                                                       u64 *__restrict__ off) {
     const u64 base = (u64)blockIdx.x * SCAN_TILE + (u64)threadIdx.x * SCAN_IPT;
     u32 v[SCAN_IPT];
@@ -298,7 +299,7 @@ __global__ __launch_bounds__(SCAN_WG) void k_scan_down(const u32 *__restrict__ c
 // ---------------------------------------------------------------- probes
 // plain 16-byte copy of a known size: the reference point that scales the
 // HBM byte counters of the other kernels
-__global__ __launch_bounds__(256) void k_copy16(const uint4 *__restrict__ src, uint4 *__restrict__ dst, u64 n) {
+static __global__ __launch_bounds__(256) void k_copy16(const uint4 *__restrict__ src, uint4 *__restrict__ dst, u64 n) {
     PG_GRID_LOOP(i, n) dst[i] = src[i];
 }
 
@@ -309,7 +310,7 @@ __global__ __launch_bounds__(wg_size<V>()) void k_probe_generate(const Pos *__re
     PG_GRID_LOOP(i, n) cnt[i] = generate_any<V>(b[i], t, moves + i * 256);
 }
 
-__global__ __launch_bounds__(256) void k_probe_make(const Pos *__restrict__ b, const u16 *__restrict__ mv, u64 n,
+static __global__ __launch_bounds__(256) void k_probe_make(const Pos *__restrict__ b, const u16 *__restrict__ mv, u64 n,
                                                     Pos *__restrict__ out, Key128 *keys, Tables t) {
     PG_GRID_LOOP(i, n) {
         Pos p = b[i];

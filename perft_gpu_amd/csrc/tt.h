@@ -1,0 +1,229 @@
+// tt.h -- device transposition table for perft, and the kernels of the
+// breadth-first perft that uses it.
+//
+// One HBM allocation, zeroed at creation, split into one table per remaining
+// depth r (the reference keeps one table per depth too, launcher.h:69-123).
+// Index = key.lo & mask; every entry verifies the full 128-bit key:
+//  * shallow entries (r <= 4), 16 bytes, the reference's HashKey128b scheme
+//    (perft_bb.h:890-892): w0 = (key.lo & ~mask) | perft, w1 = key.hi ^ w0.
+//    The index bits hold the value, so only perft < 2^bits is stored.
+//  * deep entries (r >= 5), 32 bytes (the reference packs 24, chess.h:474):
+//    {key.lo ^ v, key.hi ^ v, v, ~(key.lo ^ key.hi ^ v)}.
+// Writers use plain 16-byte stores without locks; a torn or concurrently
+// overwritten entry fails the XOR check and reads as a miss, so a hit is
+// always the value stored for that key.
+#pragma once
+#include "kernels.h"
+
+namespace pg {
+
+constexpr int TT_MAX_DEPTH = 16;
+constexpr int TT_SHALLOW_MAX = 4;
+
+struct TTView {
+    u8 *base[TT_MAX_DEPTH];  // null = no table for that remaining depth
+    u64 mask[TT_MAX_DEPTH];  // slots - 1
+};
+
+__device__ __forceinline__ bool tt_probe(const TTView &tt, int r, const Key128 &k, u64 *v) {
+    u8 *b = tt.base[r];
+    if (!b) return false;
+    const u64 m = tt.mask[r];
+    const u64 slot = k.lo & m;
+    if (r <= TT_SHALLOW_MAX) {
+        const ulonglong2 e = *reinterpret_cast<const ulonglong2 *>(b + slot * 16);
+        if ((e.y ^ e.x) == k.hi && (e.x & ~m) == (k.lo & ~m)) {
+            *v = e.x & m;
+            return true;
+        }
+        return false;
+    }
+    const ulonglong2 *p = reinterpret_cast<const ulonglong2 *>(b + slot * 32);
+    const ulonglong2 e0 = *reinterpret_cast<const ulonglong2 *>(p);
+    const ulonglong2 e1 = *reinterpret_cast<const ulonglong2 *>(p + 1);
+    const u64 val = e1.x;
+    if ((e0.x ^ val) == k.lo && (e0.y ^ val) == k.hi && e1.y == ~(k.lo ^ k.hi ^ val)) {
+        *v = val;
+        return true;
+    }
+    return false;
+}
+
+__device__ __forceinline__ bool tt_store(const TTView &tt, int r, const Key128 &k, u64 v) {
+    u8 *b = tt.base[r];
+    if (!b) return false;
+    const u64 m = tt.mask[r];
+    const u64 slot = k.lo & m;
+    if (r <= TT_SHALLOW_MAX) {
+        if (v > m) return false;  // value does not fit the index bits
+        ulonglong2 e;
+        e.x = (k.lo & ~m) | v;
+        e.y = k.hi ^ e.x;
+        *reinterpret_cast<ulonglong2 *>(b + slot * 16) = e;
+        return true;
+    }
+    ulonglong2 *p = reinterpret_cast<ulonglong2 *>(b + slot * 32);
+    ulonglong2 e0, e1;
+    e0.x = k.lo ^ v;
+    e0.y = k.hi ^ v;
+    e1.x = v;
+    e1.y = ~(k.lo ^ k.hi ^ v);
+    p[0] = e0;
+    p[1] = e1;
+    return true;
+}
+
+// child state after expansion
+enum : u8 { ST_LIVE = 0, ST_HIT = 1, ST_DUP = 2 };
+
+// counters: [r][0] probes, [r][1] hits, [r][2] stores, [r][3] duplicates
+struct TTCounters {
+    unsigned long long c[TT_MAX_DEPTH][4];
+};
+
+__device__ __forceinline__ void wave_count(unsigned long long *dst, bool pred) {
+    const u64 m = __ballot(pred);
+    if ((threadIdx.x & 63) == __ffsll((long long)__ballot(1)) - 1 && m) atomicAdd(dst, (unsigned long long)__popcll(m));
+}
+
+// child = make(parent, move) with the incremental key; probe the table of the
+// child's remaining depth rc; a hit resolves the child (its board is marked
+// invalid so it expands to nothing), a miss counts the child's moves.
+template <int V>
+__global__ __launch_bounds__(wg_size<V>()) void k_expand_tt(const Pos *__restrict__ parents, const Key128 *__restrict__ pkeys,
+                                                   const u32 *__restrict__ pidx, const u16 *__restrict__ moves, u64 m,
+                                                   Pos *__restrict__ children, Key128 *__restrict__ ckeys,
+                                                   u64 *__restrict__ cperft, u32 *__restrict__ cnt, u8 *__restrict__ state,
+                                                   u32 *__restrict__ dup, TTView tt, int rc, TTCounters *ctr, Tables t) {
+    stage_tables<V>(t);
+    PG_GRID_LOOP(i, m) {
+        const u32 pi = pidx[i];
+        Pos p = parents[pi];
+        Key128 h = pkeys[pi];
+        make_any_hash(p, moves[i], h, t);
+        u64 v = 0;
+        const bool probe = rc >= 2 && tt.base[rc];
+        const bool hit = probe && tt_probe(tt, rc, h, &v);
+        if (ctr) {
+            wave_count(&ctr->c[rc][0], probe);
+            wave_count(&ctr->c[rc][1], hit);
+        }
+        if (hit) {
+            p.white = 0;
+            cnt[i] = 0;
+            state[i] = ST_HIT;
+        } else {
+            cnt[i] = count_any<V>(p, t);
+            state[i] = ST_LIVE;
+        }
+        children[i] = p;
+        ckeys[i] = h;
+        cperft[i] = v;
+        dup[i] = (u32)i;
+    }
+}
+
+PG_HD bool same_board(const Pos &a, const Pos &b) {
+    const u64 clk = ~(0x7FULL << 56);  // the half-move clock is not part of a position
+    return a.white == b.white && (a.pawns & clk) == (b.pawns & clk) && a.knights == b.knights &&
+           a.bishopQueens == b.bishopQueens && a.rookQueens == b.rookQueens && a.kings == b.kings;
+}
+
+// In-level duplicate removal: each live child inserts its index into an
+// open-addressing set keyed by its Zobrist key; an equal board already in
+// the set becomes its owner and the child is resolved from it later.  The
+// set has at least twice as many slots as children, so a probe sequence
+// always meets an empty slot; the loop is also capped at the set size.
+static __global__ __launch_bounds__(256) void k_dedup(Pos *__restrict__ children, const Key128 *__restrict__ ckeys, u64 m,
+                                               u32 *__restrict__ cnt, u8 *__restrict__ state, u32 *__restrict__ dup,
+                                               u32 *set, u64 setMask, u32 *err, TTCounters *ctr, int rc) {
+    PG_GRID_LOOP(i, m) {
+        bool isDup = false;
+        if (state[i] == ST_LIVE) {
+            const Pos me = children[i];
+            u64 s = (ckeys[i].lo >> 7) & setMask;
+            for (u64 step = 0;; step++) {
+                if (step > setMask) { *err = 3; break; }
+                u32 cur = atomicCAS(&set[s], 0u, (u32)i + 1);
+                if (cur == 0) break;  // claimed: this child owns the position
+                const u32 o = cur - 1;
+                if (ckeys[o].lo == ckeys[i].lo && ckeys[o].hi == ckeys[i].hi && same_board(children[o], me)) {
+                    dup[i] = o;
+                    isDup = true;
+                    break;
+                }
+                s = (s + 1) & setMask;
+            }
+        }
+        if (ctr) wave_count(&ctr->c[rc][3], isDup);
+        if (isDup) {
+            state[i] = ST_DUP;
+            cnt[i] = 0;
+            children[i].white = 0;
+        }
+    }
+}
+
+// After the children's perfts are known: duplicates copy their owner's value,
+// live children are stored in the table.
+static __global__ __launch_bounds__(256) void k_resolve_store(const Key128 *__restrict__ ckeys, u64 m, u64 *cperft,
+                                                       const u8 *__restrict__ state, const u32 *__restrict__ dup,
+                                                       TTView tt, int rc, TTCounters *ctr) {
+    PG_GRID_LOOP(i, m) {
+        const u8 st = state[i];
+        bool stored = false;
+        if (st == ST_DUP) cperft[i] = cperft[dup[i]];
+        else if (st == ST_LIVE && rc >= 2) stored = tt_store(tt, rc, ckeys[i], cperft[i]);
+        if (ctr) wave_count(&ctr->c[rc][2], stored);
+    }
+}
+
+// P[i] = sum of the children's perfts of board i (children are contiguous);
+// boards resolved before expansion keep their value.
+static __global__ __launch_bounds__(256) void k_segsum(const Pos *__restrict__ boards, u64 n, const u64 *__restrict__ off,
+                                                const u64 *__restrict__ cperft, u64 *__restrict__ P) {
+    PG_GRID_LOOP(i, n) {
+        if (!pos_valid(boards[i])) continue;
+        u64 s = 0;
+        for (u64 j = off[i]; j < off[i + 1]; j++) s += cperft[j];
+        P[i] = s;
+    }
+}
+
+// probe (store = 0) or store (store = 1) a single key, e.g. the root
+static __global__ void k_tt_single(TTView tt, int r, Key128 k, u64 *val, u32 *hit, int store) {
+    if (threadIdx.x || blockIdx.x) return;
+    if (store) {
+        tt_store(tt, r, k, *val);
+    } else {
+        u64 v = 0;
+        const bool h = tt_probe(tt, r, k, &v);
+        *hit = h ? 1u : 0u;
+        *val = v;
+    }
+}
+
+// fills n 16-byte words with splitmix64 output (test hook: garbage entries)
+static __global__ __launch_bounds__(256) void k_fill_random(ulonglong2 *p, u64 n, u64 seed) {
+    PG_GRID_LOOP(i, n) {
+        u64 z = seed + (2 * i + 1) * 0x9E3779B97F4A7C15ULL;
+        z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ULL;
+        z = (z ^ (z >> 27)) * 0x94D049BB133111EBULL;
+        u64 w = z ^ (z >> 31);
+        z = w * 0x9E3779B97F4A7C15ULL + i;
+        z = (z ^ (z >> 29)) * 0xBF58476D1CE4E5B9ULL;
+        ulonglong2 e;
+        e.x = w;
+        e.y = z ^ (z >> 32);
+        p[i] = e;
+    }
+}
+
+// perft(1) of valid boards = their move counts
+static __global__ __launch_bounds__(256) void k_perft1(const Pos *__restrict__ boards, u64 n, const u32 *__restrict__ cnt, u64 *P) {
+    PG_GRID_LOOP(i, n) {
+        if (pos_valid(boards[i])) P[i] = cnt[i];
+    }
+}
+
+}  // namespace pg

@@ -1,0 +1,88 @@
+"""Transposition-table perft: exact against the no-TT path, the reference's
+values and published CPW counts; garbage in the table is never taken for a
+hit; levels split when the arena is small."""
+import numpy as np
+import pytest
+
+from conftest import golden
+from perft_gpu_amd import workloads as W
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def tt(dev, pg):
+    t = pg.TranspositionTable(dev, 2 << 30)
+    yield t
+    t.close()
+
+
+def test_tt_layout(tt):
+    nbytes, bits = tt.info()
+    assert nbytes <= 2 << 30
+    assert bits[2] > bits[3] > bits[4] > 0 and all(b > 0 for b in bits[5:])
+
+
+@pytest.mark.parametrize("v", [0, 2, 3])
+def test_tt_start_position(dev, tt, v):
+    tt.clear()
+    for d in range(1, 9):
+        print("tt start d%d variant %d" % (d, v), flush=True)
+        assert tt.perft(W.CPW["start"], d, v) == W.CPW_COUNTS["start"][d]
+    c = tt.counters()
+    assert c[2:, 1].sum() > 0          # hits
+    assert c[2:, 3].sum() > 0          # in-level duplicates
+
+
+def test_tt_without_dedup(dev, tt):
+    tt.clear()
+    dev.set_dedup(False)
+    try:
+        assert tt.perft(W.CPW["start"], 7, 0) == W.CPW_COUNTS["start"][7]
+        assert tt.counters()[:, 3].sum() == 0
+    finally:
+        dev.set_dedup(True)
+
+
+def test_tt_cpw_and_random(dev, tt, pg):
+    tt.clear()
+    for name, item in golden("cpw.json").items():
+        for d, want in enumerate(item["perft"]):
+            assert tt.perft(item["fen"], d, 0) == want, (name, d)
+    for item in golden("c2.json")["positions"]:
+        print("tt c2", item["fen"], flush=True)
+        assert tt.perft(pg.pos_from_words(item["words"]), 6, 0) == item["perft"][6]
+
+
+def test_tt_reused_across_calls_and_poison_rejected(dev, tt, pg):
+    tt.poison(12345)
+    assert tt.perft(W.CPW["kiwipete"], 5, 0) == W.CPW_COUNTS["kiwipete"][5]
+    tt.poison(999)
+    for item in golden("random.json")[:64]:
+        assert tt.perft(pg.pos_from_words(item["words"]), 4, 0) == item["perft"][4]
+    # warm table: the second call is served from it
+    tt.clear()
+    a = tt.perft(W.CPW["pos3"], 7, 0)
+    hits_before = tt.counters()[:, 1].sum()
+    b = tt.perft(W.CPW["pos3"], 7, 0)
+    assert a == b == W.CPW_COUNTS["pos3"][7]
+    assert tt.counters()[:, 1].sum() >= hits_before
+
+
+def test_tt_small_arena_splits(pg):
+    with pg.Device(0, arena_bytes=96 << 20) as small:
+        with pg.TranspositionTable(small, 256 << 20) as t:
+            print("tt split arena start d8", flush=True)
+            assert t.perft(W.CPW["start"], 8, 0) == W.CPW_COUNTS["start"][8]
+            assert small.last_stats()["splits"] > 0
+
+
+@pytest.mark.slow
+def test_tt_depth9_equals_no_tt_path(dev, pg):
+    # C4: TT perft(9) bit-exact to the plain breadth-first path and to CPW
+    with pg.TranspositionTable(dev, 16 << 30) as big:
+        print("tt d9", flush=True)
+        a = big.perft(W.CPW["start"], 9, 0)
+    print("no-tt d9", flush=True)
+    b = dev.perft(W.CPW["start"], 9, 0)
+    assert a == b == W.CPW_COUNTS["start"][9]
