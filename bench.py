@@ -9,7 +9,13 @@ positions (start position + 10..60 uniform-random legal plies) to depth 6 --
 Prints one JSON line: leaves/s over the timed steps, per-variant timings,
 the leaf kernel's roofline figures (HIP-event timed launches) and the CPU
 baseline (the reference's own CPU perft compiled from its sources, timed on
-this host on a bounded sample).
+this host on a bounded sample).  Further legs, reported beside the headline:
+  batch  4096 seeded random positions at depth 5 in one graph launch
+         (autotuned), against the one-position-per-launch loop;
+  tt     transposition-table perft of the start position at depth 9 and 10
+         with 16 GB and 64 GB tables;
+  serve  served queries (512 seeded positions, depths 3-7 drawn 40/30/20/8/2%)
+         at Poisson rates 100..10000/s: leaves/s and p50/p99 latency.
 """
 import argparse
 import json
@@ -61,6 +67,99 @@ def run_step(dev, jobs, variant, leaf_acc=None):
     return leaves
 
 
+def golden(name):
+    with open(os.path.join(ROOT, "tests", "golden", name)) as f:
+        return json.load(f)
+
+
+def leg_batch(pg, dev):
+    import numpy as np
+    from perft_gpu_amd import autotune
+    c3 = golden("c3.json")
+    arr = np.array(c3["words"], np.uint64)
+    want = np.array(c3["perft"], np.uint64)
+    total = int(want.sum())
+    space = {"slider": [0, 1, 2, 3], "dedup": [True, False], "blocks_per_cu": [4, 8], "chunk": [None]}
+    cfg, table = autotune.tune(dev, arr, 5, space=space, sample=512, log=log)
+    runs = []
+    for _ in range(3):
+        t = time.perf_counter()
+        got, st = dev.perft_batch(arr, 5, cfg["slider"], cfg["dedup"], cfg["blocks_per_cu"])
+        runs.append((time.perf_counter() - t, st))
+        if not np.array_equal(got, want):
+            raise SystemExit("MISMATCH in the 4096-position batch")
+    dt, st = min(runs, key=lambda r: r[0])
+    t = time.perf_counter()
+    s = 0
+    for w in arr:
+        s += dev.perft(pg.pos_from_words(w), 5, cfg["slider"])
+    loop = time.perf_counter() - t
+    if s != total:
+        raise SystemExit("MISMATCH in the one-position-per-launch loop")
+    return {"workload": "4096 random positions (ply 5-80) at depth 5, one graph launch", "leaves": total,
+            "ms": dt * 1e3, "graph_ms": st["graph_ms"], "leaves_per_s": total / dt, "launches": st["launches"],
+            "autotuned": {"slider": pg.SLIDER_NAMES[cfg["slider"]], "dedup": cfg["dedup"],
+                          "blocks_per_cu": cfg["blocks_per_cu"]},
+            "autotune_table": [{"slider": pg.SLIDER_NAMES[c["slider"]], "dedup": c["dedup"],
+                                "blocks_per_cu": c["blocks_per_cu"], "ms": t_ * 1e3} for c, t_ in table],
+            "one_position_per_launch": {"ms": loop * 1e3, "leaves_per_s": total / loop},
+            "speedup_vs_plain_loop": loop / dt, "exact": True}
+
+
+def leg_tt(pg, dev):
+    from perft_gpu_amd import workloads as W
+    out = []
+    for gb in (16, 64):
+        with pg.TranspositionTable(dev, gb << 30) as tt:
+            for d in (9, 10):
+                t = time.perf_counter()
+                n = tt.perft(pg.START_FEN, d, 0)
+                dt = time.perf_counter() - t
+                if n != W.CPW_COUNTS["start"][d]:
+                    raise SystemExit("MISMATCH TT perft(%d)" % d)
+                out.append({"tt_gb": gb, "depth": d, "perft": n, "s": dt, "effective_leaves_per_s": n / dt})
+                log("tt %d GB d%d %.3f s" % (gb, d, dt))
+    return {"workload": "start position, TT perft(9) and perft(10), tables cleared per size", "runs": out,
+            "exact": True}
+
+
+def leg_serve(pg, dev):
+    import numpy as np
+    from perft_gpu_amd.serve import Server, serve, poisson_arrivals, latency_summary
+    c5 = golden("c5.json")
+    pos0 = np.array(c5["words"], np.uint64)
+    dep0 = np.array(c5["depths"], np.uint32)
+    want0 = np.array(c5["perft"], np.uint64)
+    t = time.perf_counter()
+    s = 0
+    for w, d in zip(pos0, dep0):
+        s += dev.perft(pg.pos_from_words(w), int(d), 0)
+    plain = time.perf_counter() - t
+    if s != int(want0.sum()):
+        raise SystemExit("MISMATCH in plain served-query loop")
+    rates = []
+    with pg.TranspositionTable(dev, 8 << 30) as tt, Server(dev, slots=4, slot_bytes=16 << 30, tt=tt) as srv:
+        for rate in (100, 300, 1000, 3000, 10000):
+            n = int(min(2000, max(200, rate * 2)))
+            idx = np.arange(n) % len(pos0)
+            arr = poisson_arrivals(n, float(rate), seed=rate)
+            r = serve(srv, pos0[idx], dep0[idx], arr)
+            if not np.array_equal(r["results"], want0[idx]):
+                raise SystemExit("MISMATCH in served results at rate %d" % rate)
+            leaves = int(want0[idx].sum())
+            row = {"rate_per_s": rate, "queries": n, "elapsed_s": r["elapsed"], "leaves_per_s": leaves / r["elapsed"],
+                   "batches": r["batches"], "mean_batch": float(np.mean(r["batch_sizes"])),
+                   "max_busy_slots": r["max_busy_slots"]}
+            row.update(latency_summary(r["latency"]))
+            rates.append(row)
+            log("serve rate %d: p50 %.2f ms p99 %.2f ms %.1f G leaves/s" % (rate, row["p50_ms"], row["p99_ms"],
+                                                                          row["leaves_per_s"] / 1e9))
+    return {"workload": "512 seeded queries (ply 0-60; depth 3/4/5/6/7 at 40/30/20/8/2%), cycled; Poisson arrivals; "
+                        "4 slots, shared 8 GB TT", "rates": rates,
+            "plain_sequential": {"queries": len(pos0), "s": plain, "leaves_per_s": int(want0.sum()) / plain},
+            "exact": True}
+
+
 def cpu_model():
     try:
         out = subprocess.run(["lscpu"], capture_output=True, text=True).stdout
@@ -106,6 +205,7 @@ def main():
     ap.add_argument("--variant", type=int, default=-2, help="slider variant; -2 = pick the fastest")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-threads", type=int, default=min(16, os.cpu_count() or 1))
+    ap.add_argument("--legs", default="batch,tt,serve", help="extra legs beside the headline (comma list or none)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU: this build runs perft on a single MI355X "
@@ -152,6 +252,10 @@ def main():
         valu["achieved_lane_ops"] = items_per_s * VALU_INSTS_PER_ITEM * 64
         valu["frac"] = valu["achieved_lane_ops"] / valu["peak_lane_ops"]
 
+    legs = {}
+    for leg in [x for x in args.legs.split(",") if x and x != "none"]:
+        legs[leg] = {"batch": leg_batch, "tt": leg_tt, "serve": leg_serve}[leg](pg, dev)
+
     cpu = None
     if not args.no_cpu:
         cpu = cpu_baseline(jobs, args.cpu_threads)
@@ -166,6 +270,11 @@ def main():
                    "slider": pg.SLIDER_NAMES[best], "exact": True},
         "variants": variants, "roofline": roofline, "valu": valu, "cpu_baseline": cpu,
     }
+    line.update(legs)
+    if "serve" in legs:
+        r1k = [r for r in legs["serve"]["rates"] if r["rate_per_s"] == 1000][0]
+        line["served_p50_ms"] = r1k["p50_ms"]
+        line["served_p99_ms"] = r1k["p99_ms"]
     dev.close()
     print(json.dumps(line))
 

@@ -88,6 +88,9 @@ int pg_pos_to_fen(const pg_pos48 *pos, char *buf, int buflen);
 int pg_zobrist128(const pg_pos48 *pos, pg_key128 *out);
 /* MoveGeneratorBitboard::generateMoves on the host (:2471); up to 218 moves. */
 int pg_host_generate_moves(const pg_pos48 *pos, pg_move *out, uint32_t *n);
+/* MoveGeneratorBitboard::countMoves (:3010) of n boards on the host, with
+ * one slider variant (the same source the kernels compile). */
+int pg_host_count_moves(const pg_pos48 *pos, uint32_t n, int slider, uint32_t *out);
 /* MoveGeneratorBitboard::makeMove<chance, updateHash> (:3575); key may be NULL. */
 int pg_host_make_move(pg_pos48 *pos, pg_move move, pg_key128 *key);
 /* Slider lookups of one variant on the host tables (the device copies are
@@ -156,6 +159,46 @@ int pg_tt_poison(pg_tt *tt, uint64_t seed);
 /* FIND_DUPLICATES_IN_BFS (switches.h:52): 1 (default) collapses identical
  * positions within a level before expanding them. */
 int pg_set_dedup(pg_ctx *ctx, int on);
+
+/* ------------------------------------------------------------- batches */
+
+typedef struct pg_batch_stats {
+    double total_ms;        /* host wall time of the call */
+    double graph_ms;        /* HIP-event time of the graph launches */
+    uint64_t launches;      /* graph launches (1 unless the batch had to split) */
+    uint64_t splits;        /* halvings forced by the region budget */
+    uint64_t leaf_items;    /* (parent, move) items of the leaf level */
+    uint64_t graphs_cached; /* instantiated graphs held by the context */
+} pg_batch_stats;
+
+/* Batched perft (no reference counterpart beyond launcher.h:966-1179's
+ * sibling batches): perft(depth) of n roots, mixed side to move allowed, in
+ * one breadth-first search over all roots, launched as a single HIP graph
+ * (captured on first use per shape, then replayed).  dedup: collapse
+ * identical positions within a level; blocks_per_cu: persistent grid size
+ * (0 = context default); tt: optional shared table.  out[i] = perft of
+ * pos[i]. */
+int pg_perft_batch(pg_ctx *ctx, const pg_pos48 *pos, uint32_t n, uint32_t depth, int slider, int dedup,
+                   int blocks_per_cu, pg_tt *tt, uint64_t *out, pg_batch_stats *stats);
+
+/* ------------------------------------------------------------- serving */
+
+/* No reference counterpart.  A server holds `slots` batch slots, each with
+ * its own stream, graph cache and HBM region of bytes_per_slot (carved from
+ * free HBM at creation), sharing the optional table tt.  submit() stages up
+ * to max_roots roots of one depth on an idle slot and launches the batch
+ * asynchronously; poll() returns 1 and fills out[] (submission order) when
+ * it is done, 0 while it runs.  Batches on different slots overlap. */
+typedef struct pg_server pg_server;
+int pg_server_create(pg_ctx *ctx, uint32_t slots, uint64_t bytes_per_slot, uint32_t max_roots, pg_tt *tt,
+                     pg_server **out);
+void pg_server_destroy(pg_server *srv);
+int pg_server_submit(pg_server *srv, uint32_t slot, const pg_pos48 *pos, uint32_t n, uint32_t depth, int slider,
+                     int dedup);
+int pg_server_poll(pg_server *srv, uint32_t slot, uint64_t *out, pg_batch_stats *stats);
+int pg_server_slots(pg_server *srv, uint32_t *slots, uint64_t *bytes_per_slot);
+/* hipMemGetInfo of the context's device (admission by free HBM). */
+int pg_device_memory(pg_ctx *ctx, uint64_t *free_bytes, uint64_t *total_bytes);
 
 /* Plain device-to-device copy of `bytes` (16-byte lanes), `iters` times;
  * *ms gets the mean launch time.  No reference counterpart: it calibrates

@@ -56,6 +56,14 @@ class Config(ctypes.Structure):
                 ("blocks_per_cu", ctypes.c_int), ("timing", ctypes.c_int), ("leaf_mode", ctypes.c_int)]
 
 
+class BatchStats(ctypes.Structure):
+    _fields_ = [("total_ms", ctypes.c_double), ("graph_ms", ctypes.c_double), ("launches", u64), ("splits", u64),
+                ("leaf_items", u64), ("graphs_cached", u64)]
+
+    def as_dict(self):
+        return {n: getattr(self, n) for n, _ in self._fields_}
+
+
 class Stats(ctypes.Structure):
     _fields_ = [("total_ms", ctypes.c_double), ("leaf_kernel_ms", ctypes.c_double),
                 ("leaf_launches", u64), ("leaf_items", u64), ("leaf_parents", u64), ("leaves", u64),
@@ -95,6 +103,7 @@ def lib():
         "pg_zobrist128": (ctypes.c_int, [P(Pos48), P(Key128)]),
         "pg_host_generate_moves": (ctypes.c_int, [P(Pos48), P(ctypes.c_uint16), P(ctypes.c_uint32)]),
         "pg_host_make_move": (ctypes.c_int, [P(Pos48), ctypes.c_uint16, P(Key128)]),
+        "pg_host_count_moves": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_uint32, ctypes.c_int, ctypes.c_void_p]),
         "pg_host_slider_attacks": (ctypes.c_int, [ctypes.c_int, ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p,
                                                   ctypes.c_uint32, ctypes.c_void_p]),
         "pg_random_positions": (ctypes.c_int, [u64, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint32,
@@ -124,6 +133,9 @@ def lib():
         "pg_perft_tt": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, P(Pos48), ctypes.c_uint32, ctypes.c_int,
                                        P(u64)]),
         "pg_set_dedup": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int]),
+        "pg_perft_batch": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_uint32, ctypes.c_uint32,
+                                          ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p,
+                                          P(BatchStats)]),
     }
     for name, (res, args) in sig.items():
         f = getattr(L, name)
@@ -174,6 +186,13 @@ def generate_moves_host(pos):
     n = ctypes.c_uint32()
     _check(lib().pg_host_generate_moves(ctypes.byref(pos), buf, ctypes.byref(n)), "pg_host_generate_moves")
     return list(buf[:n.value])
+
+
+def count_moves_host(positions, slider):
+    arr = positions_array(positions)
+    out = np.zeros(len(arr), dtype=np.uint32)
+    _check(lib().pg_host_count_moves(_ptr(arr), len(arr), slider, _ptr(out)), "pg_host_count_moves")
+    return out
 
 
 def make_move_host(pos, move, key=None):
@@ -271,6 +290,15 @@ class Device:
         s = Stats()
         _check(lib().pg_last_stats(self._h, ctypes.byref(s)), "pg_last_stats")
         return s.as_dict()
+
+    def perft_batch(self, positions, depth, slider=SLIDER_AUTO, dedup=True, blocks_per_cu=0, tt=None):
+        """perft(depth) of every position in one graph launch; returns (counts, stats)."""
+        arr = positions_array(positions)
+        out = np.zeros(len(arr), dtype=np.uint64)
+        st = BatchStats()
+        _check(lib().pg_perft_batch(self._h, _ptr(arr), len(arr), depth, slider, 1 if dedup else 0, blocks_per_cu,
+                                    tt._h if tt is not None else None, _ptr(out), ctypes.byref(st)), "pg_perft_batch")
+        return out, st.as_dict()
 
     def count_moves(self, positions, slider=SLIDER_AUTO):
         arr = positions_array(positions)

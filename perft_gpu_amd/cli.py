@@ -1,0 +1,57 @@
+"""Command line of the reference's perft.cu (perft.cu:14-164):
+
+    python -m perft_gpu_amd.cli <fen> <depth> [numGPUs] [launchDepth] [--tt GB] [--slider N]
+
+Runs perft for every depth from 3 (minDepth, perft.cu:72) up to <depth> and
+prints the reference's result line for each (launcher.h:1496-1498).  argv[3]
+is the GPU count (the reference's usage text calls it launchdepth,
+perft.cu:82); only one GPU is used here.  argv[4], the launch depth, is
+accepted and ignored: the level loop sizes its own splits.  With --tt the
+transposition-table path runs (the reference's default build,
+switches.h:39).
+"""
+import argparse
+import sys
+import time
+
+import perft_gpu_amd as pg
+
+
+def main(argv=None):
+    ap = argparse.ArgumentParser(prog="perft_gpu_amd.cli")
+    ap.add_argument("fen", nargs="?", default=None)
+    ap.add_argument("depth", nargs="?", type=int, default=3)
+    ap.add_argument("num_gpus", nargs="?", type=int, default=1)
+    ap.add_argument("launch_depth", nargs="?", type=int, default=None)
+    ap.add_argument("--tt", type=float, default=0.0, help="transposition table size in GB (0 = no TT)")
+    ap.add_argument("--slider", type=int, default=pg.SLIDER_AUTO)
+    ap.add_argument("--min-depth", type=int, default=3)
+    args = ap.parse_args(argv)
+    if args.fen is None or len(args.fen) <= 5:
+        print("\nUsage perft_gpu_amd.cli <fen> <depth> [<numGPUs>] [<launchdepth>]")
+        print("\nAs no paramaters were provided... running default test")
+        fen = pg.START_FEN
+    else:
+        fen = args.fen
+    if args.num_gpus != 1:
+        print("\nUsing 1 GPU (this engine runs on a single MI355X)")
+    pos = pg.read_fen_string(fen)
+    print("No of GPUs detected: 1")
+    print(pg.pos_to_fen(pos))
+    with pg.Device(0) as dev:
+        tt = pg.TranspositionTable(dev, int(args.tt * (1 << 30))) if args.tt > 0 else None
+        try:
+            for d in range(min(args.min_depth, args.depth), args.depth + 1):
+                t0 = time.perf_counter()
+                res = tt.perft(pos, d, args.slider) if tt else dev.perft(pos, d, args.slider)
+                dt = time.perf_counter() - t0
+                print("\nGPU Perft %d: %d,   Time taken: %g seconds, nps: %d" % (d, res, dt, int(res / dt) if dt else 0))
+                sys.stdout.flush()
+        finally:
+            if tt:
+                tt.close()
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

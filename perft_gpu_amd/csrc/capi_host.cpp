@@ -161,6 +161,26 @@ int pg_host_generate_moves(const pg_pos48 *pos, pg_move *out, uint32_t *n) {
     return PG_OK;
 }
 
+int pg_host_count_moves(const pg_pos48 *pos, uint32_t n, int slider, uint32_t *out) {
+    const HostTables *h = tables_or_null();
+    if (!h) return PG_ETABLE;
+    if (n && (!pos || !out)) return PG_EINVAL;
+    if (slider < 0 || slider >= SL_COUNT) return PG_EINVAL;
+    const Tables t = h->view();
+    for (uint32_t i = 0; i < n; i++) {
+        Pos p;
+        memcpy(&p, &pos[i], sizeof(p));
+        if (!pos_sane(p)) return PG_EINVAL;
+        switch (slider) {
+        case SL_KOGGE: out[i] = count_any<SL_KOGGE>(p, t); break;
+        case SL_PLAIN: out[i] = count_any<SL_PLAIN>(p, t); break;
+        case SL_FANCY: out[i] = count_any<SL_FANCY>(p, t); break;
+        default: out[i] = count_any<SL_BYTE>(p, t); break;
+        }
+    }
+    return PG_OK;
+}
+
 int pg_host_make_move(pg_pos48 *pos, pg_move move, pg_key128 *key) {
     const HostTables *h = tables_or_null();
     if (!h) return PG_ETABLE;

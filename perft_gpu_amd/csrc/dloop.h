@@ -1,0 +1,372 @@
+// dloop.h -- device-driven level loop: the breadth-first perft of a batch of
+// roots with no host round trip between levels, so the whole search can be
+// captured once into a HIP graph and replayed.
+//
+// Sizes and array pointers of every level live in a Plan in device memory.
+// Single-thread planning kernels read the scan total, carve the next level's
+// arrays out of a bump region (stack discipline: transient move/parent/set
+// arrays are released after expansion) and publish the level's size; the
+// work kernels are persistent grid-stride loops that read their size from
+// the plan.  If a level does not fit, the planner raises `overflow` and
+// publishes size 0, every later kernel becomes a no-op, and the host reruns
+// the batch in halves.  Per-root results come back up the levels by
+// segmented sums (children of a board are contiguous), as in the TT path.
+#pragma once
+#include "tt.h"
+
+namespace pg {
+
+struct DLevel {
+    u64 n;            // boards at this level
+    Pos *b;
+    Key128 *k;
+    u32 *cnt;
+    u64 *off;         // n + 1 exclusive offsets of the children
+    u64 *perft;
+    u8 *state;
+    u32 *dup;
+    u64 *tiles;       // scan tile sums
+    u16 *mv;          // transient: moves / parents of the children
+    u32 *par;
+    u32 *set;         // transient: duplicate set
+    u64 setMask;
+    u64 release;      // bump top to return to after expansion
+};
+
+struct Plan {
+    u8 *base;
+    u64 cap, top;
+    u32 overflow, err;
+    u64 leafItems;
+    u64 acc;
+    DLevel lv[TT_MAX_DEPTH];
+};
+
+__device__ __forceinline__ void *plan_alloc(Plan *P, u64 bytes) {
+    const u64 a = (P->top + 255) & ~255ULL;
+    if (a + bytes > P->cap) {
+        P->overflow = 1;
+        return nullptr;
+    }
+    P->top = a + bytes;
+    return P->base + a;
+}
+
+#define PG_GRID_LOOP_N(i, n) PG_GRID_LOOP(i, n)
+
+// level-0 move counts (deeper levels get them from expansion)
+template <int V>
+__global__ __launch_bounds__(wg_size<V>()) void kd_count(Plan *P, Tables t) {
+    stage_tables<V>(t);
+    const DLevel L = P->lv[0];
+    PG_GRID_LOOP(i, L.n) L.cnt[i] = pos_valid(L.b[i]) ? count_any<V>(L.b[i], t) : 0;
+}
+
+// roots: probe the table of depth D; hits are resolved before expansion
+static __global__ __launch_bounds__(256) void kd_root_probe(Plan *P, TTView tt, int D, TTCounters *ctr) {
+    const DLevel L = P->lv[0];
+    PG_GRID_LOOP(i, L.n) {
+        u64 v = 0;
+        const bool probe = D >= 2 && tt.base[D] && pos_valid(L.b[i]);
+        const bool hit = probe && tt_probe(tt, D, L.k[i], &v);
+        if (ctr) {
+            wave_count(&ctr->c[D][0], probe);
+            wave_count(&ctr->c[D][1], hit);
+        }
+        L.state[i] = hit ? ST_HIT : ST_LIVE;
+        L.perft[i] = v;
+        if (hit) {
+            L.b[i].white = 0;
+            L.cnt[i] = 0;
+        }
+    }
+}
+
+static __global__ __launch_bounds__(256) void kd_root_store(Plan *P, TTView tt, int D, TTCounters *ctr) {
+    const DLevel L = P->lv[0];
+    if (P->overflow) return;
+    PG_GRID_LOOP(i, L.n) {
+        const bool st = D >= 2 && L.state[i] == ST_LIVE && tt_store(tt, D, L.k[i], L.perft[i]);
+        if (ctr) wave_count(&ctr->c[D][2], st);
+    }
+}
+
+// allocate the offsets and scan tiles of level l
+static __global__ void kd_plan_scan(Plan *P, int l) {
+    if (threadIdx.x || blockIdx.x) return;
+    DLevel &L = P->lv[l];
+    L.off = nullptr;
+    L.tiles = nullptr;
+    if (L.n == 0) return;
+    L.off = (u64 *)plan_alloc(P, (L.n + 1) * 8);
+    L.tiles = L.off ? (u64 *)plan_alloc(P, ((L.n + SCAN_TILE - 1) / SCAN_TILE) * 8) : nullptr;
+    if (!L.tiles) L.n = 0;
+}
+
+static __global__ __launch_bounds__(SCAN_WG) void kd_scan_reduce(Plan *P, int l) {
+    const DLevel L = P->lv[l];
+    const u64 tiles = (L.n + SCAN_TILE - 1) / SCAN_TILE;
+    for (u64 tile = blockIdx.x; tile < tiles; tile += gridDim.x) {
+        const u64 base = tile * SCAN_TILE + (u64)threadIdx.x * SCAN_IPT;
+        u32 v[SCAN_IPT];
+        load_tile(L.cnt, L.n, base, v);
+        u64 s = 0;
+#pragma unroll
+        for (int j = 0; j < SCAN_IPT; j++) s += v[j];
+        u64 tot;
+        block_excl_scan<SCAN_WG>(s, &tot);
+        if (threadIdx.x == 0) L.tiles[tile] = tot;
+    }
+}
+
+static __global__ __launch_bounds__(1024) void kd_scan_spine(Plan *P, int l) {
+    const DLevel L = P->lv[l];
+    if (L.n == 0) return;
+    const u64 tiles = (L.n + SCAN_TILE - 1) / SCAN_TILE;
+    u64 carry = 0;
+    for (u64 s = 0; s < tiles; s += 1024) {
+        const u64 i = s + threadIdx.x;
+        const u64 v = i < tiles ? L.tiles[i] : 0;
+        u64 tot;
+        const u64 ex = block_excl_scan<1024>(v, &tot);
+        if (i < tiles) L.tiles[i] = carry + ex;
+        carry += tot;
+    }
+    if (threadIdx.x == 0) L.off[L.n] = carry;
+}
+
+static __global__ __launch_bounds__(SCAN_WG) void kd_scan_down(Plan *P, int l) {
+    const DLevel L = P->lv[l];
+    const u64 tiles = (L.n + SCAN_TILE - 1) / SCAN_TILE;
+    for (u64 tile = blockIdx.x; tile < tiles; tile += gridDim.x) {
+        const u64 base = tile * SCAN_TILE + (u64)threadIdx.x * SCAN_IPT;
+        u32 v[SCAN_IPT];
+        load_tile(L.cnt, L.n, base, v);
+        u64 s = 0;
+#pragma unroll
+        for (int j = 0; j < SCAN_IPT; j++) s += v[j];
+        u64 tot;
+        u64 run = L.tiles[tile] + block_excl_scan<SCAN_WG>(s, &tot);
+#pragma unroll
+        for (int j = 0; j < SCAN_IPT; j++) {
+            if (base + j < L.n) L.off[base + j] = run;
+            run += v[j];
+        }
+    }
+}
+
+// allocate level l+1 (children of l) and the transient arrays
+static __global__ void kd_plan_children(Plan *P, int l, int wantKeys, int wantSet) {
+    if (threadIdx.x || blockIdx.x) return;
+    DLevel &L = P->lv[l];
+    DLevel &C = P->lv[l + 1];
+    memset(&C, 0, sizeof(C));
+    const u64 m = L.n ? L.off[L.n] : 0;
+    if (m == 0 || m > 0xFFFFFFFFULL) {
+        if (m) P->overflow = 1;
+        return;
+    }
+    C.b = (Pos *)plan_alloc(P, m * sizeof(Pos));
+    C.cnt = (u32 *)plan_alloc(P, m * 4);
+    C.perft = (u64 *)plan_alloc(P, m * 8);
+    C.state = (u8 *)plan_alloc(P, m);
+    C.dup = (u32 *)plan_alloc(P, m * 4);
+    if (wantKeys) C.k = (Key128 *)plan_alloc(P, m * sizeof(Key128));
+    L.release = P->top;
+    L.mv = (u16 *)plan_alloc(P, m * 2);
+    L.par = (u32 *)plan_alloc(P, m * 4);
+    if (wantSet) {
+        u64 s = 1;
+        while (s < 2 * m) s <<= 1;
+        L.set = (u32 *)plan_alloc(P, s * 4);
+        L.setMask = s - 1;
+    }
+    if (P->overflow) {
+        memset(&C, 0, sizeof(C));
+        return;
+    }
+    C.n = m;
+}
+
+static __global__ void kd_release(Plan *P, int l) {
+    if (threadIdx.x || blockIdx.x) return;
+    DLevel &L = P->lv[l];
+    if (P->lv[l + 1].n) P->top = L.release;
+    L.mv = nullptr;
+    L.par = nullptr;
+    L.set = nullptr;
+}
+
+template <int V>
+__global__ __launch_bounds__(wg_size<V>()) void kd_generate(Plan *P, int l, Tables t) {
+    stage_tables<V>(t);
+    const DLevel L = P->lv[l];
+    if (P->lv[l + 1].n == 0) return;
+    PG_GRID_LOOP(i, L.n) {
+        const u64 o = L.off[i];
+        const u32 room = (u32)(L.off[i + 1] - o);
+        if (room == 0) continue;
+        const u32 c = generate_any<V>(L.b[i], t, L.mv + o, room);
+        if (c != room) P->err = 1;
+        for (u32 j = 0; j < room; j++) L.par[o + j] = (u32)i;
+    }
+}
+
+// children: make (with key), probe the child depth's table, count moves
+template <int V>
+__global__ __launch_bounds__(wg_size<V>()) void kd_expand(Plan *P, int l, TTView tt, int rc, TTCounters *ctr, Tables t) {
+    stage_tables<V>(t);
+    const DLevel L = P->lv[l];
+    const DLevel C = P->lv[l + 1];
+    PG_GRID_LOOP(i, C.n) {
+        const u32 pi = L.par[i];
+        Pos p = L.b[pi];
+        u64 v = 0;
+        bool hit = false;
+        if (C.k) {
+            Key128 h = L.k[pi];
+            make_any_hash(p, L.mv[i], h, t);
+            C.k[i] = h;
+            const bool probe = rc >= 2 && tt.base[rc];
+            hit = probe && tt_probe(tt, rc, h, &v);
+            if (ctr) {
+                wave_count(&ctr->c[rc][0], probe);
+                wave_count(&ctr->c[rc][1], hit);
+            }
+        } else {
+            make_any(p, L.mv[i], t);
+        }
+        if (hit) {
+            p.white = 0;
+            C.cnt[i] = 0;
+            C.state[i] = ST_HIT;
+        } else {
+            C.cnt[i] = count_any<V>(p, t);
+            C.state[i] = ST_LIVE;
+        }
+        C.b[i] = p;
+        C.perft[i] = v;
+        C.dup[i] = (u32)i;
+    }
+}
+
+// fill kernel for the duplicate set (graphs get no memset nodes)
+static __global__ __launch_bounds__(256) void kd_clear_set(Plan *P, int l) {
+    const DLevel L = P->lv[l];
+    if (!L.set || P->lv[l + 1].n == 0) return;
+    PG_GRID_LOOP(i, L.setMask + 1) L.set[i] = 0;
+}
+
+static __global__ __launch_bounds__(256) void kd_dedup(Plan *P, int l, TTCounters *ctr, int rc) {
+    const DLevel L = P->lv[l];
+    const DLevel C = P->lv[l + 1];
+    if (!L.set) return;
+    PG_GRID_LOOP(i, C.n) {
+        bool isDup = false;
+        if (C.state[i] == ST_LIVE) {
+            const Pos me = C.b[i];
+            u64 s = (C.k[i].lo >> 7) & L.setMask;
+            for (u64 step = 0;; step++) {
+                if (step > L.setMask) { P->err = 3; break; }
+                const u32 cur = atomicCAS(&L.set[s], 0u, (u32)i + 1);
+                if (cur == 0) break;
+                const u32 o = cur - 1;
+                if (C.k[o].lo == C.k[i].lo && C.k[o].hi == C.k[i].hi && same_board(C.b[o], me)) {
+                    C.dup[i] = o;
+                    isDup = true;
+                    break;
+                }
+                s = (s + 1) & L.setMask;
+            }
+        }
+        if (ctr) wave_count(&ctr->c[rc][3], isDup);
+        if (isDup) {
+            C.state[i] = ST_DUP;
+            C.cnt[i] = 0;
+            C.b[i].white = 0;
+        }
+    }
+}
+
+// perft(2) of the boards of level l, fused (see k_leaf_fused)
+template <int V>
+__global__ __launch_bounds__(LEAF_WG) void kd_leaf(Plan *P, int l, u32 *err, Tables t) {
+    const DLevel L = P->lv[l];
+    __shared__ Pos sb[LEAF_WG];
+    __shared__ u16 smv[LEAF_CAP];
+    __shared__ u8 sown[LEAF_CAP];
+    __shared__ u32 sper[LEAF_WG];
+    __shared__ u32 snext;
+    const int tid = threadIdx.x;
+    for (u64 tile = blockIdx.x; tile * LEAF_WG < L.n; tile += gridDim.x) {
+        const u64 i = tile * LEAF_WG + tid;
+        const bool live = i < L.n;
+        const u32 c = live ? L.cnt[i] : 0;
+        if (live) sb[tid] = L.b[i];
+        sper[tid] = 0;
+        u64 total64;
+        const u32 pos = (u32)block_excl_scan<LEAF_WG>(c, &total64);
+        const u32 total = (u32)total64;
+        if (tid == 0) atomicAdd(&P->leafItems, total64);
+        u32 start = 0;
+        while (start < total) {
+            const u32 end = start + LEAF_CAP;
+            if (tid == 0) snext = total;
+            __syncthreads();
+            const bool here = c && pos >= start && pos + c <= end;
+            if (here) {
+                const u32 g = generate_any<V>(sb[tid], t, smv + (pos - start), c);
+                if (g != c) *err = 2;
+                for (u32 j = 0; j < c; j++) sown[pos - start + j] = (u8)tid;
+            } else if (c && pos >= start) {
+                atomicMin(&snext, pos);
+            }
+            __syncthreads();
+            const u32 wend = snext;
+            for (u32 j = start + tid; j < wend; j += LEAF_WG) {
+                const u32 k = j - start;
+                const int b = sown[k];
+                atomicAdd(&sper[b], count_after<V>(sb[b], smv[k], t));
+            }
+            __syncthreads();
+            start = wend;
+        }
+        if (live && pos_valid(sb[tid])) L.perft[i] = sper[tid];
+        __syncthreads();
+    }
+}
+
+static __global__ __launch_bounds__(256) void kd_perft1(Plan *P, int l) {
+    const DLevel L = P->lv[l];
+    PG_GRID_LOOP(i, L.n) {
+        if (pos_valid(L.b[i])) L.perft[i] = L.cnt[i];
+    }
+}
+
+// level l+1 after its perfts are known: duplicates take their owner's value,
+// live children are stored (remaining depth rc)
+static __global__ __launch_bounds__(256) void kd_resolve(Plan *P, int l, TTView tt, int rc, TTCounters *ctr) {
+    const DLevel C = P->lv[l + 1];
+    if (P->overflow) return;
+    PG_GRID_LOOP(i, C.n) {
+        const u8 st = C.state[i];
+        bool stored = false;
+        if (st == ST_DUP) C.perft[i] = C.perft[C.dup[i]];
+        else if (st == ST_LIVE && C.k && rc >= 2) stored = tt_store(tt, rc, C.k[i], C.perft[i]);
+        if (ctr) wave_count(&ctr->c[rc][2], stored);
+    }
+}
+
+static __global__ __launch_bounds__(256) void kd_segsum(Plan *P, int l) {
+    const DLevel L = P->lv[l];
+    const DLevel C = P->lv[l + 1];
+    if (P->overflow) return;
+    PG_GRID_LOOP(i, L.n) {
+        if (!pos_valid(L.b[i])) continue;
+        u64 s = 0;
+        for (u64 j = L.off[i]; j < L.off[i + 1]; j++) s += C.perft[j];
+        L.perft[i] = s;
+    }
+}
+
+}  // namespace pg

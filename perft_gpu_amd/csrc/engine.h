@@ -28,6 +28,7 @@ void set_error(const char *fmt, ...);
 
 
 struct TTable;
+struct Slot;
 
 struct Engine {
     int device = 0, cus = 0, bpc = 8, slider = SL_KOGGE;
@@ -59,12 +60,22 @@ struct Engine {
     int split(int v, const Pos *boards, u64 n, const u32 *cnt, int r, int ply);
     int perft(const Pos &root, u32 depth, int v, u64 *out);
 
+    // batched searches (graph.hip)
+    Slot *batchSlot = nullptr;
+
     // transposition-table perft
     bool dedup = true;
     int tt_bfs(int v, TTable &tt, Pos *B, Key128 *K, u64 n, u32 *cnt, int r, u64 *P, int ply);
     int tt_split(int v, TTable &tt, Pos *B, Key128 *K, u64 n, u32 *cnt, int r, u64 *P, int ply);
     int perft_tt(TTable &tt, const Pos &root, u32 depth, int v, u64 *out);
 };
+
+int slot_init(Engine &e, Slot &sl, u8 *region, size_t bytes, hipStream_t stream);
+void slot_free(Slot &sl);
+void slot_delete(Slot *sl);
+void slot_forget_tt(Slot &sl, const TTable *tt);  // drop graphs that bake this table in
+int batch_perft(Engine &e, Slot &sl, TTable *tt, const Pos *roots, u64 n, u32 depth, int v, int dedup, int bpc,
+                u64 *out, pg_batch_stats *bs);
 
 // Device transposition table (the object behind pg_tt).
 struct TTable {

@@ -310,7 +310,7 @@ int pg_init(int device, const pg_config *cfg, pg_ctx **out) {
     (void)hipMemGetInfo(&freeB, &totalB);
     size_t want = c->cfg.arena_bytes ? c->cfg.arena_bytes : (48ull << 30);
     if (!c->cfg.arena_bytes && want > freeB / 2) want = freeB / 2;
-    if (want < (64ull << 20)) want = 64ull << 20;
+    if (want < (4ull << 20)) want = 4ull << 20;
     if (hipMalloc(&c->arena, want) != hipSuccess) {
         set_error("arena reservation of %zu MiB failed", want >> 20);
         return fail(PG_ENOMEM);
@@ -326,6 +326,8 @@ void pg_shutdown(pg_ctx *c) {
     if (!c) return;
     (void)hipSetDevice(c->device);
     if (c->stream) (void)hipStreamSynchronize(c->stream);
+    slot_delete(c->batchSlot);
+    c->batchSlot = nullptr;
     if (c->arena) (void)hipFree(c->arena);
     if (c->dTables) (void)hipFree(c->dTables);
     if (c->dSmall) (void)hipFree(c->dSmall);

@@ -307,8 +307,11 @@ PG_HD int castle_count(const Pos &p, const MoveCtx &x) {
 }
 
 // ---------------------------------------------------------------- count
+template <int C> PG_HD u32 count_moves_ks(const Pos &p, const Tables &t);
+
 template <int C, int V>
 PG_HD u32 count_moves(const Pos &p, const Tables &t) {
+    if constexpr (V == SL_KOGGE) return count_moves_ks<C>(p, t);
     MoveCtx x;
     prologue<C, V>(p, t, x);
     const u64 tgt = x.target, empty = ~x.occ, np = ~x.pinned;
@@ -352,6 +355,163 @@ PG_HD u32 count_moves(const Pos &p, const Tables &t) {
             n += popc(m);
         }
     }
+    n += castle_count<C>(p, x);
+    n += ep_moves<C>(p, t, x, nullptr);
+    return (u32)n;
+}
+
+// ---------------------------------------------------------------- count, Kogge-Stone
+// The Kogge-Stone count in one pass over the eight directions, written to
+// keep the VALU instruction count down:
+//  * each direction's propagator chain (empty, empty & empty<<s, ...) is
+//    built once and shared by the enemy-attack fill and the own-move fill;
+//  * enemy sliders fill through empty squares only; the one square the
+//    moving king must not step to because a slider checks it through the
+//    king ("behind" it on the check line) is added explicitly;
+//  * east rays of every slider at once by subtraction: with the h-file as
+//    sentinel, o ^ (o - 2s) sets exactly the squares from each slider up to
+//    its first blocker; west rays the same on the bit-reversed board.
+PG_HD u64 rev64(u64 x) {
+#if defined(__clang__)
+    return __builtin_bitreverse64(x);
+#else
+    x = ((x >> 1) & 0x5555555555555555ULL) | ((x & 0x5555555555555555ULL) << 1);
+    x = ((x >> 2) & 0x3333333333333333ULL) | ((x & 0x3333333333333333ULL) << 2);
+    x = ((x >> 4) & 0x0F0F0F0F0F0F0F0FULL) | ((x & 0x0F0F0F0F0F0F0F0FULL) << 4);
+    return __builtin_bswap64(x);
+#endif
+}
+
+// squares east of each slider in s up to and including its first blocker;
+// every slider must be in occ
+PG_HD u64 east_of(u64 s, u64 occ) {
+    const u64 o = occ | FILEH;
+    return o ^ (o - ((s & ~FILEH) << 1));
+}
+
+struct Prop3 {
+    u64 a, b, c;
+};
+template <int S, u64 M> PG_HD Prop3 prop_up(u64 e) {
+    Prop3 p;
+    p.a = e & M;
+    p.b = p.a & (p.a << S);
+    p.c = p.b & (p.b << (2 * S));
+    return p;
+}
+template <int S, u64 M> PG_HD Prop3 prop_dn(u64 e) {
+    Prop3 p;
+    p.a = e & M;
+    p.b = p.a & (p.a >> S);
+    p.c = p.b & (p.b >> (2 * S));
+    return p;
+}
+template <int S, u64 M> PG_HD u64 fill_up_p(u64 g, const Prop3 &p) {
+    g |= (g << S) & p.a;
+    g |= (g << (2 * S)) & p.b;
+    g |= (g << (4 * S)) & p.c;
+    return (g << S) & M;
+}
+template <int S, u64 M> PG_HD u64 fill_dn_p(u64 g, const Prop3 &p) {
+    g |= (g >> S) & p.a;
+    g |= (g >> (2 * S)) & p.b;
+    g |= (g >> (4 * S)) & p.c;
+    return (g >> S) & M;
+}
+
+template <int C>
+PG_HD u32 count_moves_ks(const Pos &p, const Tables &t) {
+    MoveCtx x;
+    const u64 pawns = pos_pawns(p);
+    x.occ = p.kings | pawns | p.knights | p.bishopQueens | p.rookQueens;
+    const u64 black = x.occ & ~p.white;
+    x.mine = C == WHITE ? p.white : black;
+    x.enemy = C == WHITE ? black : p.white;
+    x.myPawns = pawns & x.mine;
+    x.myKing = p.kings & x.mine;
+    x.k = ctz(x.myKing) & 63;
+    const int k = x.k;
+    const u64 eP = pawns & x.enemy, eN = p.knights & x.enemy, eK = p.kings & x.enemy;
+    const u64 eB = p.bishopQueens & x.enemy, eR = p.rookQueens & x.enemy;
+    x.eRooks = eR;
+
+    u64 cand = (t.rookEmpty[k] & eR) | (t.bishopEmpty[k] & eB);
+    u64 pinned = 0, chk = 0, behind = 0;
+    for (; cand; cand &= cand - 1) {
+        const int s = ctz(cand);
+        const u64 btw = t.between[(k << 6) | s];
+        const u64 b = btw & x.occ;
+        if (b == 0) {
+            chk |= bit(s);
+            behind |= t.line[(s << 6) | k] & t.king[k] & ~(btw | bit(s));
+        } else if (!several(b)) {
+            pinned |= b;
+        }
+    }
+    chk |= (t.knight[k] & eN) | (pawn_att<C>(x.myKing) & eP);
+    x.pinned = pinned & x.mine;
+    x.checkers = chk;
+    u64 tgt = ~x.mine;
+    if (chk) tgt &= several(chk) ? 0ULL : (chk | t.between[(k << 6) | ctz(chk)]);
+    x.target = tgt;
+
+    const u64 empty = ~x.occ, np = ~x.pinned;
+    const u64 rq = p.rookQueens & x.mine, bq = p.bishopQueens & x.mine;
+    const u64 fk = file_of(k), rk = rank_of(k), dk = diag_of(k), ak = anti_of(k);
+    const u64 rf = rq & (np | fk), rr = rq & (np | rk), bd = bq & (np | dk), ba = bq & (np | ak);
+    u64 att = pawn_att<1 - C>(eP) | knight_att(eN) | king_att(eK) | behind;
+    int n = 0;
+    {
+        const Prop3 e = prop_up<8, ~0ULL>(empty);
+        att |= fill_up_p<8, ~0ULL>(eR, e);
+        n += popc(fill_up_p<8, ~0ULL>(rf, e) & tgt);
+    }
+    {
+        const Prop3 e = prop_dn<8, ~0ULL>(empty);
+        att |= fill_dn_p<8, ~0ULL>(eR, e);
+        n += popc(fill_dn_p<8, ~0ULL>(rf, e) & tgt);
+    }
+    att |= east_of(eR, x.occ);
+    n += popc(east_of(rr, x.occ) & tgt);
+    {
+        const u64 occR = rev64(x.occ);
+        att |= rev64(east_of(rev64(eR), occR));
+        n += popc(east_of(rev64(rr), occR) & rev64(tgt));
+    }
+    {
+        const Prop3 e = prop_up<9, ~FILEA>(empty);
+        att |= fill_up_p<9, ~FILEA>(eB, e);
+        n += popc(fill_up_p<9, ~FILEA>(bd, e) & tgt);
+    }
+    {
+        const Prop3 e = prop_dn<9, ~FILEH>(empty);
+        att |= fill_dn_p<9, ~FILEH>(eB, e);
+        n += popc(fill_dn_p<9, ~FILEH>(bd, e) & tgt);
+    }
+    {
+        const Prop3 e = prop_up<7, ~FILEH>(empty);
+        att |= fill_up_p<7, ~FILEH>(eB, e);
+        n += popc(fill_up_p<7, ~FILEH>(ba, e) & tgt);
+    }
+    {
+        const Prop3 e = prop_dn<7, ~FILEA>(empty);
+        att |= fill_dn_p<7, ~FILEA>(eB, e);
+        n += popc(fill_dn_p<7, ~FILEA>(ba, e) & tgt);
+    }
+    x.threatened = att;
+    n += popc(t.king[k] & ~x.mine & ~att);
+
+    // pawns
+    const u64 PROMO = RANK1 | RANK8;
+    const u64 p1 = fwd<C>(x.myPawns & (np | fk)) & empty;
+    const u64 p2 = fwd<C>(p1 & (C == WHITE ? RANK3 : RANK6)) & empty & tgt;
+    const u64 t1 = p1 & tgt;
+    n += popc(t1) + 3 * popc(t1 & PROMO) + popc(p2);
+    const u64 cw = fwd_west<C>(x.myPawns & (np | (C == WHITE ? ak : dk))) & x.enemy & tgt;
+    const u64 ce = fwd_east<C>(x.myPawns & (np | (C == WHITE ? dk : ak))) & x.enemy & tgt;
+    n += popc(cw) + 3 * popc(cw & PROMO) + popc(ce) + 3 * popc(ce & PROMO);
+    // knights (pinned knights never move)
+    for (u64 b = p.knights & x.mine & np; b; b &= b - 1) n += popc(t.knight[ctz(b)] & tgt);
     n += castle_count<C>(p, x);
     n += ep_moves<C>(p, t, x, nullptr);
     return (u32)n;

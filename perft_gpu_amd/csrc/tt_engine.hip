@@ -269,6 +269,8 @@ int pg_tt_clear(pg_tt *t) {
 void pg_tt_destroy(pg_tt *t) {
     if (!t) return;
     (void)hipSetDevice(t->eng->device);
+    (void)hipStreamSynchronize(t->eng->stream);
+    if (t->eng->batchSlot) slot_forget_tt(*t->eng->batchSlot, t);
     (void)hipFree(t->mem);
     (void)hipFree(t->dCtr);
     delete t;

@@ -1,0 +1,87 @@
+"""Batched perft (one graph launch over many roots): each result equals the
+single-position result and the reference's value; graph replays are exact
+with eager work in between; an arena too small for the batch splits it."""
+import numpy as np
+import pytest
+
+from conftest import golden
+from perft_gpu_amd import workloads as W
+
+pytestmark = pytest.mark.gpu
+
+
+def random_set(n=256):
+    items = golden("random.json")[:n]
+    return np.array([it["words"] for it in items], np.uint64), items
+
+
+@pytest.mark.parametrize("v", [0, 1, 2, 3])
+@pytest.mark.parametrize("dedup", [False, True])
+def test_batch_depth4_matches_reference(dev, v, dedup):
+    arr, items = random_set()
+    print("batch d4 variant", v, "dedup", dedup, flush=True)
+    got, st = dev.perft_batch(arr, 4, v, dedup)
+    assert [int(x) for x in got] == [it["perft"][4] for it in items]
+    assert st["launches"] == 1
+
+
+@pytest.mark.parametrize("depth", [0, 1, 2, 3])
+def test_batch_shallow_depths(dev, depth):
+    arr, items = random_set(64)
+    got, _ = dev.perft_batch(arr, depth, 0)
+    assert [int(x) for x in got] == [it["perft"][depth] for it in items]
+
+
+def test_batch_mixed_sides_and_duplicates(dev, pg):
+    # the same position several times, both colours interleaved
+    arr, items = random_set(32)
+    idx = np.array([3, 3, 0, 7, 3, 1, 0, 5, 6, 2, 2], np.int64)
+    got, _ = dev.perft_batch(arr[idx], 4, 0, True)
+    assert [int(x) for x in got] == [items[i]["perft"][4] for i in idx]
+
+
+def test_graph_replays_with_eager_work_between(dev, pg):
+    c2 = golden("c2.json")["positions"]
+    arr = np.array([x["words"] for x in c2], np.uint64)
+    want = [x["perft"][5] for x in c2]
+    for rep in range(3):
+        print("replay", rep, flush=True)
+        got, st = dev.perft_batch(arr, 5, 0, True)
+        assert [int(x) for x in got] == want
+        # eager single-position work on the same context between replays
+        assert dev.perft(W.CPW["kiwipete"], 4, 0) == W.CPW_COUNTS["kiwipete"][4]
+
+
+def test_batch_equals_single_position_path(dev, pg):
+    c2 = golden("c2.json")["positions"]
+    arr = np.array([x["words"] for x in c2], np.uint64)
+    got, _ = dev.perft_batch(arr, 6, 0, True)
+    for g, x in zip(got, c2):
+        assert int(g) == x["perft"][6] == dev.perft(pg.pos_from_words(x["words"]), 6, 0)
+
+
+def test_batch_splits_when_region_small(pg):
+    arr, items = random_set(128)
+    with pg.Device(0, arena_bytes=8 << 20) as small:
+        got, st = small.perft_batch(arr, 4, 0, True)
+        assert [int(x) for x in got] == [it["perft"][4] for it in items]
+        assert st["splits"] > 0
+
+
+def test_batch_with_shared_tt(dev, pg):
+    arr, items = random_set(128)
+    with pg.TranspositionTable(dev, 1 << 30) as tt:
+        for rep in range(2):
+            got, _ = dev.perft_batch(arr, 4, 0, True, tt=tt)
+            assert [int(x) for x in got] == [it["perft"][4] for it in items]
+        assert tt.counters()[4, 1] > 0   # second pass hits the roots
+
+
+@pytest.mark.slow
+def test_c3_batch_4096_positions_depth5(dev):
+    c3 = golden("c3.json")
+    arr = np.array(c3["words"], np.uint64)
+    print("c3 batch", len(arr), flush=True)
+    got, st = dev.perft_batch(arr, 5, 0, True)
+    assert [int(x) for x in got] == c3["perft"]
+    assert st["launches"] == 1

@@ -58,6 +58,38 @@ def test_host_movegen_matches_reference(pg):
         assert sorted(key(m) for m in mv) == sorted(key(m) for m in item["moves"])
 
 
+@pytest.mark.parametrize("v", [0, 1, 2, 3])
+def test_host_count_moves_every_variant(pg, orc, v):
+    # positions of all kinds: random games, their children and grandchildren
+    items = golden("movegen.json")
+    boards = [tuple(it["words"]) for it in items]
+    for w in list(boards[:96]):
+        kids = orc.generate_boards(w)
+        boards.extend(kids)
+        for kid in kids[:3]:
+            boards.extend(orc.generate_boards(kid))
+    for fen in W.CPW.values():
+        boards.append(orc.fen(fen))
+    arr = np.array(boards, np.uint64)
+    got = pg.count_moves_host(arr, v)
+    want = np.array([orc.count_moves(b) for b in boards], np.uint32)
+    assert np.array_equal(got, want), np.nonzero(got != want)[0][:5]
+
+
+def test_host_kogge_stone_random_occupancies(pg):
+    # per-square Kogge-Stone attacks against a ray walk at random occupancies
+    rng = np.random.default_rng(5)
+    sq = np.arange(64, dtype=np.uint8)
+    for trial in range(300):
+        occ = int(rng.integers(0, 2**63, dtype=np.int64)) & int(rng.integers(0, 2**63, dtype=np.int64))
+        occ |= 1 << int(rng.integers(0, 64))
+        # rook attacks from each occupied square, via the host KS lookup
+        rooks = [s for s in range(64) if occ >> s & 1]
+        got = pg.slider_attacks_host(0, 1, np.array(rooks, np.uint8), np.full(len(rooks), occ, np.uint64))
+        for s, a in zip(rooks, got):
+            assert int(a) == _ks(s, occ, True)
+
+
 def test_host_make_move_matches_reference(pg):
     for item in golden("zobrist.json"):
         p = pg.pos_from_words(item["words"])
