@@ -386,6 +386,28 @@ class TranspositionTable:
         return int(out.value)
 
 
+def divide(dev, pos, depth, slider=SLIDER_AUTO, tt=None):
+    """Divided perft (launcher.h:1344-1351, 1392-1450): perft(depth-1) of every
+    root move, all children in one batched GPU search.  Returns {move: count}."""
+    if isinstance(pos, str):
+        pos = read_fen_string(pos)
+    moves = generate_moves_host(pos)
+    if depth <= 1:
+        return {m: 1 for m in moves}
+    kids = np.array([make_move_host(pos, m)[0].words() for m in moves], dtype=np.uint64).reshape(-1, 6)
+    counts, _ = dev.perft_batch(kids, depth - 1, slider, True, 0, tt)
+    return {m: int(c) for m, c in zip(moves, counts)}
+
+
+def move_to_str(m):
+    """CMove -> long algebraic (e2e4, e7e8q)."""
+    fr, to, fl = m & 63, (m >> 6) & 63, m >> 12
+    s = "%s%d%s%d" % ("abcdefgh"[fr & 7], (fr >> 3) + 1, "abcdefgh"[to & 7], (to >> 3) + 1)
+    if fl & 8:
+        s += "nbrq"[fl & 3]
+    return s
+
+
 def init_gpu(device=0, **kw):
     """initGPU (launcher.h:1506): select the GPU, upload tables, reserve the arena."""
     return Device(device, **kw)

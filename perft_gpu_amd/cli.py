@@ -26,6 +26,7 @@ def main(argv=None):
     ap.add_argument("--tt", type=float, default=0.0, help="transposition table size in GB (0 = no TT)")
     ap.add_argument("--slider", type=int, default=pg.SLIDER_AUTO)
     ap.add_argument("--min-depth", type=int, default=3)
+    ap.add_argument("--divide", action="store_true", help="print perft(depth-1) per root move at the last depth")
     args = ap.parse_args(argv)
     if args.fen is None or len(args.fen) <= 5:
         print("\nUsage perft_gpu_amd.cli <fen> <depth> [<numGPUs>] [<launchdepth>]")
@@ -47,6 +48,11 @@ def main(argv=None):
                 dt = time.perf_counter() - t0
                 print("\nGPU Perft %d: %d,   Time taken: %g seconds, nps: %d" % (d, res, dt, int(res / dt) if dt else 0))
                 sys.stdout.flush()
+            if args.divide:
+                div = pg.divide(dev, pos, args.depth, args.slider, tt)
+                for m, c in div.items():
+                    print("%s: %d" % (pg.move_to_str(m), c))
+                print("total: %d" % sum(div.values()))
         finally:
             if tt:
                 tt.close()

@@ -290,7 +290,7 @@ static __global__ __launch_bounds__(256) void kd_dedup(Plan *P, int l, TTCounter
 
 // perft(2) of the boards of level l, fused (see k_leaf_fused)
 template <int V>
-__global__ __launch_bounds__(LEAF_WG) void kd_leaf(Plan *P, int l, u32 *err, Tables t) {
+__global__ __launch_bounds__(LEAF_WG) __attribute__((amdgpu_waves_per_eu(V == SL_KOGGE ? 4 : 5, 8))) void kd_leaf(Plan *P, int l, u32 *err, Tables t) {
     const DLevel L = P->lv[l];
     __shared__ Pos sb[LEAF_WG];
     __shared__ u16 smv[LEAF_CAP];

@@ -183,7 +183,7 @@ __device__ __forceinline__ u64 block_excl_scan(u64 v, u64 *total) {
 constexpr int LEAF_WG = 256, LEAF_CAP = 6144;
 
 template <int V, bool PER>
-__global__ __launch_bounds__(LEAF_WG) void k_leaf_fused(const Pos *__restrict__ boards, const u32 *__restrict__ cnt, u64 n,
+__global__ __launch_bounds__(LEAF_WG) __attribute__((amdgpu_waves_per_eu(V == SL_KOGGE ? 4 : 5, 8))) void k_leaf_fused(const Pos *__restrict__ boards, const u32 *__restrict__ cnt, u64 n,
                                                         u64 *acc, u64 *items, u64 *__restrict__ per, u32 *err, Tables t) {
     static_assert(V != SL_BYTE, "the byte-lookup tables leave no LDS for the move window");
     __shared__ Pos sb[LEAF_WG];

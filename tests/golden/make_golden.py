@@ -13,6 +13,9 @@ engine's seeded generator and are stored as FEN and as the 48-byte words.
   c2.json         the 16 C2 positions, perft(1..6)
   c3.json         the 4096 C3 positions, perft(5)
   c5.json         served-query positions/depths and their perft
+  edge.json       special-rule positions (en passant that uncovers check,
+                  castling through/into check, promotions out of check,
+                  double check, stalemate), perft(1..d)
 """
 import json
 import os
@@ -59,6 +62,23 @@ def main(which):
             counts = [1] + [R.perft_mt(p, d, THREADS) for d in range(1, depth_cap[name] + 1)]
             out[name] = {"fen": fen, "words": words(p), "perft": counts}
         dump("cpw.json", out)
+
+    if "edge" in which:
+        # well-known special-rule test positions (community "perft tests" set)
+        edge = [("3k4/3p4/8/K1P4r/8/8/8/8 b - - 0 1", 6), ("8/8/4k3/8/2p5/8/B2P2K1/8 w - - 0 1", 6),
+                ("8/8/1k6/2b5/2pP4/8/5K2/8 b - d3 0 1", 6), ("5k2/8/8/8/8/8/8/4K2R w K - 0 1", 6),
+                ("3k4/8/8/8/8/8/8/R3K3 w Q - 0 1", 6), ("r3k2r/1b4bq/8/8/8/8/7B/R3K2R w KQkq - 0 1", 4),
+                ("r3k2r/8/3Q4/8/8/5q2/8/R3K2R b KQkq - 0 1", 4), ("2K2r2/4P3/8/8/8/8/8/3k4 w - - 0 1", 6),
+                ("8/8/1P2K3/8/2n5/1q6/8/5k2 b - - 0 1", 5), ("4k3/1P6/8/8/8/8/K7/8 w - - 0 1", 6),
+                ("8/P1k5/K7/8/8/8/8/8 w - - 0 1", 6), ("K1k5/8/P7/8/8/8/8/8 w - - 0 1", 6),
+                ("8/k1P5/8/1K6/8/8/8/8 w - - 0 1", 7), ("8/8/2k5/5q2/5n2/8/5K2/8 b - - 0 1", 4),
+                ("rnbqkbnr/pp1ppppp/8/2pP4/8/8/PPP1PPPP/RNBQKBNR w KQkq c6 0 2", 5),
+                ("8/2p5/8/KP5r/8/8/8/7k w - - 0 1", 6)]
+        items = []
+        for fen, d in edge:
+            p = R.fen(fen)
+            items.append({"fen": fen, "words": words(p), "perft": [1] + [R.perft_mt(p, k, THREADS) for k in range(1, d + 1)]})
+        dump("edge.json", items)
 
     rnd, plies = pg.random_positions(0x5EED0001, 256, 0, 80)
     rnd = [tuple(int(x) for x in w) for w in rnd]
@@ -120,4 +140,4 @@ def main(which):
 
 
 if __name__ == "__main__":
-    main(sys.argv[1:] or ["cpw", "zobrist", "movegen", "random", "c2", "c3", "c5"])
+    main(sys.argv[1:] or ["cpw", "edge", "zobrist", "movegen", "random", "c2", "c3", "c5"])

@@ -60,3 +60,28 @@ def test_small_arena_forces_splits_and_stays_exact(pg):
             st = small.last_stats()
             assert st["splits"] > 0 and st["peak_arena"] <= 64 << 20
             assert st["nodes"][1:7] == W.CPW_COUNTS["start"][1:7]
+
+
+def test_divide_matches_oracle(dev, pg, orc):
+    p = orc.fen(W.CPW["kiwipete"])
+    want = orc.divide(p, 4)
+    got = pg.divide(dev, W.CPW["kiwipete"], 4)
+    key = lambda m: (m & 0xFFF) | ((((m >> 12) & 3) + 1 if m & 0x8000 else 0) << 12)
+    assert {key(m): c for m, c in got.items()} == {key(m): c for m, c in want.items()}
+    assert sum(got.values()) == W.CPW_COUNTS["kiwipete"][4]
+
+
+def test_cli_prints_reference_lines(pg, capsys):
+    from perft_gpu_amd import cli
+    cli.main([W.CPW["pos3"], "5"])
+    out = capsys.readouterr().out
+    for d in (3, 4, 5):
+        assert "GPU Perft %d: %d," % (d, W.CPW_COUNTS["pos3"][d]) in out
+
+
+@pytest.mark.parametrize("v", [0, 1, 2, 3])
+def test_special_rule_positions(dev, pg, v):
+    for item in golden("edge.json"):
+        p = pg.pos_from_words(item["words"])
+        for d, want in enumerate(item["perft"]):
+            assert dev.perft(p, d, v) == want, (item["fen"], d, v)

@@ -195,3 +195,20 @@ def test_workload_fixtures_are_reproduced(pg):
     c2 = golden("c2.json")
     pos, plies = W.c2_positions()
     assert [list(map(int, w)) for w in pos] == [x["words"] for x in c2["positions"]]
+
+
+@pytest.mark.parametrize("v", [0, 1, 2, 3])
+def test_host_count_special_rule_trees(pg, orc, v):
+    # every node of the special-rule positions' trees to depth 3
+    boards = []
+    frontier = [tuple(it["words"]) for it in golden("edge.json")]
+    for _ in range(3):
+        boards.extend(frontier)
+        nxt = []
+        for b in frontier:
+            nxt.extend(orc.generate_boards(b))
+        frontier = nxt
+    arr = np.array(boards, np.uint64)
+    got = pg.count_moves_host(arr, v)
+    want = np.array([orc.count_moves(b) for b in boards], np.uint32)
+    assert np.array_equal(got, want), np.nonzero(got != want)[0][:5]

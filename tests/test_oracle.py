@@ -94,3 +94,15 @@ def test_planted_mistake_is_caught(orc):
     kids = orc.generate_boards(p)
     assert sum(orc.perft(k, 3) for k in kids) == W.CPW_COUNTS["pos3"][4]
     assert sum(orc.perft(k, 3) for k in kids[1:]) != W.CPW_COUNTS["pos3"][4]
+
+
+def test_special_rule_positions(orc):
+    # en passant that uncovers check, castling through/into check,
+    # promotions out of check, double check, stalemate (reference values)
+    for item in golden("edge.json"):
+        p = orc.fen(item["fen"])
+        assert list(p) == item["words"]
+        for d, want in enumerate(item["perft"]):
+            if want > 3_000_000:
+                continue
+            assert orc.perft(p, d) == want, (item["fen"], d)
