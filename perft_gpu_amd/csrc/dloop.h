@@ -59,7 +59,7 @@ template <int V>
 __global__ __launch_bounds__(wg_size<V>()) void kd_count(Plan *P, Tables t) {
     stage_tables<V>(t);
     const DLevel L = P->lv[0];
-    PG_GRID_LOOP(i, L.n) L.cnt[i] = pos_valid(L.b[i]) ? count_any<V>(L.b[i], t) : 0;
+    PG_GRID_LOOP(i, L.n) L.cnt[i] = count_any<V>(L.b[i], t);
 }
 
 // roots: probe the table of depth D; hits are resolved before expansion
@@ -67,18 +67,17 @@ static __global__ __launch_bounds__(256) void kd_root_probe(Plan *P, TTView tt, 
     const DLevel L = P->lv[0];
     PG_GRID_LOOP(i, L.n) {
         u64 v = 0;
-        const bool probe = D >= 2 && tt.base[D] && pos_valid(L.b[i]);
+        const bool probe = D >= 2 && tt.base[D] && L.k;
         const bool hit = probe && tt_probe(tt, D, L.k[i], &v);
         if (ctr) {
             wave_count(&ctr->c[D][0], probe);
             wave_count(&ctr->c[D][1], hit);
         }
+        // the roots are left untouched so that a rerun after an overflow sees
+        // the same input; a hit is recorded in state/perft and expands to nothing
         L.state[i] = hit ? ST_HIT : ST_LIVE;
         L.perft[i] = v;
-        if (hit) {
-            L.b[i].white = 0;
-            L.cnt[i] = 0;
-        }
+        if (hit) L.cnt[i] = 0;
     }
 }
 
@@ -86,7 +85,7 @@ static __global__ __launch_bounds__(256) void kd_root_store(Plan *P, TTView tt, 
     const DLevel L = P->lv[0];
     if (P->overflow) return;
     PG_GRID_LOOP(i, L.n) {
-        const bool st = D >= 2 && L.state[i] == ST_LIVE && tt_store(tt, D, L.k[i], L.perft[i]);
+        const bool st = D >= 2 && L.k && L.state[i] == ST_LIVE && tt_store(tt, D, L.k[i], L.perft[i]);
         if (ctr) wave_count(&ctr->c[D][2], st);
     }
 }
@@ -237,7 +236,6 @@ __global__ __launch_bounds__(wg_size<V>()) void kd_expand(Plan *P, int l, TTView
             make_any(p, L.mv[i], t);
         }
         if (hit) {
-            p.white = 0;
             C.cnt[i] = 0;
             C.state[i] = ST_HIT;
         } else {
@@ -283,7 +281,6 @@ static __global__ __launch_bounds__(256) void kd_dedup(Plan *P, int l, TTCounter
         if (isDup) {
             C.state[i] = ST_DUP;
             C.cnt[i] = 0;
-            C.b[i].white = 0;
         }
     }
 }
@@ -331,7 +328,7 @@ __global__ __launch_bounds__(LEAF_WG) __attribute__((amdgpu_waves_per_eu(V == SL
             __syncthreads();
             start = wend;
         }
-        if (live && pos_valid(sb[tid])) L.perft[i] = sper[tid];
+        if (live && L.state[i] == ST_LIVE) L.perft[i] = sper[tid];
         __syncthreads();
     }
 }
@@ -339,7 +336,7 @@ __global__ __launch_bounds__(LEAF_WG) __attribute__((amdgpu_waves_per_eu(V == SL
 static __global__ __launch_bounds__(256) void kd_perft1(Plan *P, int l) {
     const DLevel L = P->lv[l];
     PG_GRID_LOOP(i, L.n) {
-        if (pos_valid(L.b[i])) L.perft[i] = L.cnt[i];
+        if (L.state[i] == ST_LIVE) L.perft[i] = L.cnt[i];
     }
 }
 
@@ -362,7 +359,7 @@ static __global__ __launch_bounds__(256) void kd_segsum(Plan *P, int l) {
     const DLevel C = P->lv[l + 1];
     if (P->overflow) return;
     PG_GRID_LOOP(i, L.n) {
-        if (!pos_valid(L.b[i])) continue;
+        if (L.state[i] != ST_LIVE) continue;
         u64 s = 0;
         for (u64 j = L.off[i]; j < L.off[i + 1]; j++) s += C.perft[j];
         L.perft[i] = s;
