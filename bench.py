@@ -235,7 +235,7 @@ def main():
     leaves = 0
     for _ in range(args.steps):
         leaves += run_step(dev, jobs, best, acc)
-    elapsed = time.perf_counter() - t0
+    elapsed = max(time.perf_counter() - t0, 1e-9)
     value = leaves / elapsed
 
     avg_ms = acc["ms"] / max(1, acc["launches"])
@@ -271,10 +271,10 @@ def main():
 
     line = {
         "metric": METRIC, "value": value, "unit": "leaves/s", "n_gpus": 1, "steps": args.steps,
-        "warmup": args.warmup, "ms_per_step": elapsed / args.steps * 1e3, "higher_is_better": True,
+        "warmup": args.warmup, "ms_per_step": elapsed / max(1, args.steps) * 1e3, "higher_is_better": True,
         "scaling": "weak", "vs_baseline": None, "dtype": "u64", "data": "synthetic",
         "config": {"workload": "plain GPU BFS perft, no TT: start position d7 + d8 and 16 seeded random "
-                               "positions (ply 10-60) at d6", "leaves_per_step": leaves // args.steps,
+                               "positions (ply 10-60) at d6", "leaves_per_step": leaves // max(1, args.steps),
                    "slider": pg.SLIDER_NAMES[best], "exact": True},
         "variants": variants, "roofline": roofline, "valu": valu, "cpu_baseline": cpu,
     }

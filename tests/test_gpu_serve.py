@@ -43,7 +43,7 @@ def test_served_batch_overflow_finishes_in_halves(dev, pg):
     from perft_gpu_amd.serve import Server
     g = golden("c2.json")["positions"]
     arr = np.array([x["words"] for x in g], np.uint64)
-    with Server(dev, slots=1, slot_bytes=64 << 20, tt=None) as srv:
+    with Server(dev, slots=1, slot_bytes=16 << 20, tt=None) as srv:
         srv.submit(0, arr, 5, list(range(len(arr))))
         r = None
         while r is None:
