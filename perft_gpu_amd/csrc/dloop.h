@@ -291,7 +291,7 @@ __global__ __launch_bounds__(LEAF_WG) __attribute__((amdgpu_waves_per_eu(V == SL
     const DLevel L = P->lv[l];
     __shared__ Pos sb[LEAF_WG];
     __shared__ u16 smv[LEAF_CAP];
-    __shared__ u8 sown[LEAF_CAP];
+    __shared__ u16 stag[LEAF_CAP];  // owner | moving piece << 8
     __shared__ u32 sper[LEAF_WG];
     __shared__ u32 snext;
     const int tid = threadIdx.x;
@@ -312,9 +312,8 @@ __global__ __launch_bounds__(LEAF_WG) __attribute__((amdgpu_waves_per_eu(V == SL
             __syncthreads();
             const bool here = c && pos >= start && pos + c <= end;
             if (here) {
-                const u32 g = generate_any<V>(sb[tid], t, smv + (pos - start), c);
+                const u32 g = generate_any<V>(sb[tid], t, smv + (pos - start), c, stag + (pos - start), (u32)tid);
                 if (g != c) *err = 2;
-                for (u32 j = 0; j < c; j++) sown[pos - start + j] = (u8)tid;
             } else if (c && pos >= start) {
                 atomicMin(&snext, pos);
             }
@@ -322,8 +321,9 @@ __global__ __launch_bounds__(LEAF_WG) __attribute__((amdgpu_waves_per_eu(V == SL
             const u32 wend = snext;
             for (u32 j = start + tid; j < wend; j += LEAF_WG) {
                 const u32 k = j - start;
-                const int b = sown[k];
-                atomicAdd(&sper[b], count_after<V>(sb[b], smv[k], t));
+                const u32 tg = stag[k];
+                const int b = tg & 255;
+                atomicAdd(&sper[b], count_after_as<V>(sb[b], smv[k], (int)(tg >> 8), t));
             }
             __syncthreads();
             start = wend;

@@ -78,6 +78,16 @@ __device__ __forceinline__ u32 count_after(Pos p, u16 mv, const Tables &t) {
     make_move<WHITE, false>(p, mv, nullptr, t);
     return count_moves<BLACK, V>(p, t);
 }
+// the same with the moving piece known
+template <int V>
+__device__ __forceinline__ u32 count_after_as(Pos p, u16 mv, int piece, const Tables &t) {
+    if (pos_side(p)) {
+        make_move_as<BLACK, false>(p, mv, piece, nullptr, t);
+        return count_moves<WHITE, V>(p, t);
+    }
+    make_move_as<WHITE, false>(p, mv, piece, nullptr, t);
+    return count_moves<BLACK, V>(p, t);
+}
 
 template <int V>
 __global__ __launch_bounds__(wg_size<V>()) void k_count(const Pos *__restrict__ b, u64 n, u32 *__restrict__ cnt, Tables t) {
@@ -188,7 +198,7 @@ __global__ __launch_bounds__(LEAF_WG) __attribute__((amdgpu_waves_per_eu(V == SL
     static_assert(V != SL_BYTE, "the byte-lookup tables leave no LDS for the move window");
     __shared__ Pos sb[LEAF_WG];
     __shared__ u16 smv[LEAF_CAP];
-    __shared__ u8 sown[LEAF_CAP];
+    __shared__ u16 stag[LEAF_CAP];  // owner | moving piece << 8
     __shared__ u32 sper[PER ? LEAF_WG : 1];
     __shared__ u32 snext;
     const int tid = threadIdx.x;
@@ -210,9 +220,8 @@ __global__ __launch_bounds__(LEAF_WG) __attribute__((amdgpu_waves_per_eu(V == SL
             __syncthreads();
             const bool here = c && pos >= start && pos + c <= end;
             if (here) {
-                const u32 g = generate_any<V>(sb[tid], t, smv + (pos - start), c);
+                const u32 g = generate_any<V>(sb[tid], t, smv + (pos - start), c, stag + (pos - start), (u32)tid);
                 if (g != c) *err = 2;
-                for (u32 j = 0; j < c; j++) sown[pos - start + j] = (u8)tid;
             } else if (c && pos >= start) {
                 atomicMin(&snext, pos);
             }
@@ -220,8 +229,9 @@ __global__ __launch_bounds__(LEAF_WG) __attribute__((amdgpu_waves_per_eu(V == SL
             const u32 wend = snext;
             for (u32 j = start + tid; j < wend; j += LEAF_WG) {
                 const u32 k = j - start;
-                const int b = sown[k];
-                const u32 m = count_after<V>(sb[b], smv[k], t);
+                const u32 tg = stag[k];
+                const int b = tg & 255;
+                const u32 m = count_after_as<V>(sb[b], smv[k], (int)(tg >> 8), t);
                 if (PER) atomicAdd(&sper[b], m);
                 else s += m;
             }

@@ -64,6 +64,33 @@ PG_HD int ctz(u64 x) { return __builtin_ctzll(x); }
 PG_HD u64 bit(int s) { return 1ULL << s; }
 PG_HD bool several(u64 x) { return (x & (x - 1)) != 0; }
 
+// Three-input bitwise function given by its truth table F: bit i of F is the
+// result for inputs (a, b, c) = (i >> 2 & 1, i >> 1 & 1, i & 1), so F for an
+// expression is that expression evaluated on a = 0xF0, b = 0xCC, c = 0xAA.
+// On gfx950 each 32-bit half is one v_bitop3_b32; the compiler does not form
+// these from 64-bit and/or chains by itself.
+template <u32 F>
+PG_HD u64 bop3(u64 a, u64 b, u64 c) {
+#if defined(__HIP_DEVICE_COMPILE__)
+    const u32 lo = __builtin_amdgcn_bitop3_b32((u32)a, (u32)b, (u32)c, F);
+    const u32 hi = __builtin_amdgcn_bitop3_b32((u32)(a >> 32), (u32)(b >> 32), (u32)(c >> 32), F);
+    return ((u64)hi << 32) | lo;
+#else
+    u64 r = 0;
+    for (int i = 0; i < 8; i++)
+        if ((F >> i) & 1) r |= ((i & 4) ? a : ~a) & ((i & 2) ? b : ~b) & ((i & 1) ? c : ~c);
+    return r;
+#endif
+}
+PG_HD u64 or_and(u64 a, u64 b, u64 c) { return bop3<0xF8>(a, b, c); }    // a | (b & c)
+PG_HD u64 and3(u64 a, u64 b, u64 c) { return bop3<0x80>(a, b, c); }      // a & b & c
+PG_HD u64 or3(u64 a, u64 b, u64 c) { return bop3<0xFE>(a, b, c); }       // a | b | c
+PG_HD u64 and_or(u64 a, u64 b, u64 c) { return bop3<0xE0>(a, b, c); }    // a & (b | c)
+PG_HD u64 and_nor(u64 a, u64 b, u64 c) { return bop3<0x10>(a, b, c); }   // a & ~b & ~c
+PG_HD u64 andn(u64 a, u64 b) { return bop3<0x30>(a, b, b); }             // a & ~b
+PG_HD u64 or_xor(u64 a, u64 b, u64 c) { return bop3<0xF6>(a, b, c); }    // a | (b ^ c)
+PG_HD u64 xor_and(u64 a, u64 b, u64 c) { return bop3<0x28>(a, b, c); }   // (a ^ b) & c
+
 // A board the generator accepts: one king per side, no white bit on an empty
 // square and an en-passant field of 0 (none) or a file 1..8.  (Inputs arriving over the C ABI are checked with this.)
 PG_HD bool pos_sane(const Pos &p) {
@@ -247,13 +274,22 @@ PG_HD void prologue(const Pos &p, const Tables &t, MoveCtx &x) {
     x.target = tgt;
 }
 
+enum { P_PAWN = 1, P_KNIGHT, P_BISHOP, P_ROOK, P_QUEEN, P_KING };
+
 // Bounded move writer: never stores past `cap`, but keeps counting so a
-// caller can detect a mismatch with the counted size.
+// caller can detect a mismatch with the counted size.  With `tag` set it also
+// stores owner | moving piece << 8 per move (the fused leaf's window), so the
+// child can be made without looking the piece up.
 struct MoveSink {
     u16 *out;
     u32 cap, n;
-    PG_HD void add(u16 m) {
-        if (n < cap) out[n] = m;
+    u16 *tag;
+    u32 owner;
+    PG_HD void add(u16 m, int piece) {
+        if (n < cap) {
+            out[n] = m;
+            if (tag) tag[n] = (u16)(owner | ((u32)piece << 8));
+        }
         n++;
     }
 };
@@ -273,7 +309,7 @@ PG_HD int ep_moves(const Pos &p, const Tables &t, const MoveCtx &x, MoveSink *ou
         if (x.checkers != cap) return 0;
         src &= ~x.pinned;
         for (; src; src &= src - 1) {
-            if (out) out->add((u16)(ctz(src) | (to << 6) | (MF_EP_CAPTURE << 12)));
+            if (out) out->add((u16)(ctz(src) | (to << 6) | (MF_EP_CAPTURE << 12)), P_PAWN);
             n++;
         }
         return n;
@@ -288,7 +324,7 @@ PG_HD int ep_moves(const Pos &p, const Tables &t, const MoveCtx &x, MoveSink *ou
             ok = ((att_e(x.eRooks, e) | att_w(x.eRooks, e)) & x.myKing) == 0;
         }
         if (ok) {
-            if (out) out->add((u16)(s | (to << 6) | (MF_EP_CAPTURE << 12)));
+            if (out) out->add((u16)(s | (to << 6) | (MF_EP_CAPTURE << 12)), P_PAWN);
             n++;
         }
     }
@@ -363,14 +399,17 @@ PG_HD u32 count_moves(const Pos &p, const Tables &t) {
 // ---------------------------------------------------------------- count, Kogge-Stone
 // The Kogge-Stone count in one pass over the eight directions, written to
 // keep the VALU instruction count down:
+//  * and/or chains are three-input bitop3 instructions (two per 64-bit word);
 //  * each direction's propagator chain (empty, empty & empty<<s, ...) is
 //    built once and shared by the enemy-attack fill and the own-move fill;
 //  * enemy sliders fill through empty squares only; the one square the
 //    moving king must not step to because a slider checks it through the
 //    king ("behind" it on the check line) is added explicitly;
-//  * east rays of every slider at once by subtraction: with the h-file as
-//    sentinel, o ^ (o - 2s) sets exactly the squares from each slider up to
-//    its first blocker; west rays the same on the bit-reversed board.
+//  * east rays of every slider at once by one add: with the h-file as
+//    sentinel, o = occ | FILEH and e = ~o, the squares from each slider up
+//    to its first blocker are o ^ (o - 2s) = e ^ (e + 2s); west rays the
+//    same on the bit-reversed board;
+//  * pinned pieces need the king-line masks only when something is pinned.
 PG_HD u64 rev64(u64 x) {
 #if defined(__clang__)
     return __builtin_bitreverse64(x);
@@ -382,60 +421,65 @@ PG_HD u64 rev64(u64 x) {
 #endif
 }
 
-// squares east of each slider in s up to and including its first blocker;
-// every slider must be in occ
-PG_HD u64 east_of(u64 s, u64 occ) {
-    const u64 o = occ | FILEH;
-    return o ^ (o - ((s & ~FILEH) << 1));
-}
+PG_HD u64 nor2(u64 a, u64 b) { return bop3<0x03>(a, b, b); }  // ~(a | b)
 
+// squares east of each slider in s up to and including its first blocker,
+// given eo = ~(occ | FILEH); every slider must be in occ
+PG_HD u64 east_sum(u64 s, u64 eo) { return (andn(s, FILEH) << 1) + eo; }
+
+template <int S> PG_HD u64 shift(u64 x) {
+#if defined(__HIP_DEVICE_COMPILE__)
+    if constexpr (S > 0 && S < 32) {
+        u64 r;
+        asm("v_lshlrev_b64 %0, %1, %2" : "=v"(r) : "i"(S), "v"(x));
+        return r;
+    } else if constexpr (S < 0 && S > -32) {
+        u64 r;
+        asm("v_lshrrev_b64 %0, %1, %2" : "=v"(r) : "i"(-S), "v"(x));
+        return r;
+    }
+#endif
+    return S > 0 ? x << S : x >> -S;
+}
 struct Prop3 {
     u64 a, b, c;
 };
-template <int S, u64 M> PG_HD Prop3 prop_up(u64 e) {
+// propagators of a direction from the empty squares that may be entered
+template <int S> PG_HD Prop3 props(u64 a) {
     Prop3 p;
-    p.a = e & M;
-    p.b = p.a & (p.a << S);
-    p.c = p.b & (p.b << (2 * S));
+    p.a = a;
+    p.b = a & shift<S>(a);
+    p.c = p.b & shift<2 * S>(p.b);
     return p;
 }
-template <int S, u64 M> PG_HD Prop3 prop_dn(u64 e) {
-    Prop3 p;
-    p.a = e & M;
-    p.b = p.a & (p.a >> S);
-    p.c = p.b & (p.b >> (2 * S));
-    return p;
+// generators grown through the propagators (the ray squares shifted back
+// by one step, still to be shifted by S and wrap-masked)
+template <int S> PG_HD u64 grow(u64 g, const Prop3 &p) {
+    g = or_and(g, shift<S>(g), p.a);
+    g = or_and(g, shift<2 * S>(g), p.b);
+    return or_and(g, shift<4 * S>(g), p.c);
 }
-template <int S, u64 M> PG_HD u64 fill_up_p(u64 g, const Prop3 &p) {
-    g |= (g << S) & p.a;
-    g |= (g << (2 * S)) & p.b;
-    g |= (g << (4 * S)) & p.c;
-    return (g << S) & M;
-}
-template <int S, u64 M> PG_HD u64 fill_dn_p(u64 g, const Prop3 &p) {
-    g |= (g >> S) & p.a;
-    g |= (g >> (2 * S)) & p.b;
-    g |= (g >> (4 * S)) & p.c;
-    return (g >> S) & M;
-}
+// the side's part of a piece word (every piece word lies inside occ)
+template <int C> PG_HD u64 side_of(u64 b, u64 white) { return C == WHITE ? b & white : andn(b, white); }
 
 template <int C>
 PG_HD u32 count_moves_ks(const Pos &p, const Tables &t) {
     MoveCtx x;
+    const u64 W = p.white;
     const u64 pawns = pos_pawns(p);
-    x.occ = p.kings | pawns | p.knights | p.bishopQueens | p.rookQueens;
-    const u64 black = x.occ & ~p.white;
-    x.mine = C == WHITE ? p.white : black;
-    x.enemy = C == WHITE ? black : p.white;
-    x.myPawns = pawns & x.mine;
-    x.myKing = p.kings & x.mine;
+    x.occ = or3(p.kings, pawns, or3(p.knights, p.bishopQueens, p.rookQueens));
+    x.mine = C == WHITE ? W : andn(x.occ, W);
+    x.enemy = C == WHITE ? andn(x.occ, W) : W;
+    x.myPawns = side_of<C>(pawns, W);
+    x.myKing = side_of<C>(p.kings, W);
     x.k = ctz(x.myKing) & 63;
     const int k = x.k;
-    const u64 eP = pawns & x.enemy, eN = p.knights & x.enemy, eK = p.kings & x.enemy;
-    const u64 eB = p.bishopQueens & x.enemy, eR = p.rookQueens & x.enemy;
+    const u64 eP = side_of<1 - C>(pawns, W), eN = side_of<1 - C>(p.knights, W);
+    const u64 eB = side_of<1 - C>(p.bishopQueens, W), eR = side_of<1 - C>(p.rookQueens, W);
     x.eRooks = eR;
 
-    u64 cand = (t.rookEmpty[k] & eR) | (t.bishopEmpty[k] & eB);
+    // sliders on a king line: no blocker -> checker, one blocker -> pinned
+    u64 cand = or_and(t.rookEmpty[k] & eR, t.bishopEmpty[k], eB);
     u64 pinned = 0, chk = 0, behind = 0;
     for (; cand; cand &= cand - 1) {
         const int s = ctz(cand);
@@ -443,75 +487,99 @@ PG_HD u32 count_moves_ks(const Pos &p, const Tables &t) {
         const u64 b = btw & x.occ;
         if (b == 0) {
             chk |= bit(s);
-            behind |= t.line[(s << 6) | k] & t.king[k] & ~(btw | bit(s));
+            behind |= and_nor(t.line[(s << 6) | k] & t.king[k], btw, bit(s));
         } else if (!several(b)) {
             pinned |= b;
         }
     }
-    chk |= (t.knight[k] & eN) | (pawn_att<C>(x.myKing) & eP);
+    {   // pawn and knight checks
+        const u64 kw = C == WHITE ? shift<7>(x.myKing) : shift<-9>(x.myKing);
+        const u64 ke = C == WHITE ? shift<9>(x.myKing) : shift<-7>(x.myKing);
+        chk = or_and(or_and(chk, or_and(andn(kw, FILEH), ke, ~FILEA), eP), t.knight[k], eN);
+    }
     x.pinned = pinned & x.mine;
     x.checkers = chk;
     u64 tgt = ~x.mine;
     if (chk) tgt &= several(chk) ? 0ULL : (chk | t.between[(k << 6) | ctz(chk)]);
     x.target = tgt;
 
-    const u64 empty = ~x.occ, np = ~x.pinned;
-    const u64 rq = p.rookQueens & x.mine, bq = p.bishopQueens & x.mine;
-    const u64 fk = file_of(k), rk = rank_of(k), dk = diag_of(k), ak = anti_of(k);
-    const u64 rf = rq & (np | fk), rr = rq & (np | rk), bd = bq & (np | dk), ba = bq & (np | ak);
-    u64 att = pawn_att<1 - C>(eP) | knight_att(eN) | king_att(eK) | behind;
+    // movers; a pinned piece moves only along its pin line
+    const u64 rq = side_of<C>(p.rookQueens, W), bq = side_of<C>(p.bishopQueens, W);
+    u64 rf = rq, rr = rq, bd = bq, ba = bq, pf = x.myPawns, pw = x.myPawns, pe = x.myPawns;
+    u64 myN = side_of<C>(p.knights, W);
+    if (x.pinned) {
+        const u64 np = ~x.pinned;
+        const u64 fk = file_of(k), dk = diag_of(k), ak = anti_of(k);
+        rf = and_or(rq, np, fk);
+        rr = and_or(rq, np, rank_of(k));
+        bd = and_or(bq, np, dk);
+        ba = and_or(bq, np, ak);
+        pf = and_or(x.myPawns, np, fk);
+        pw = and_or(x.myPawns, np, C == WHITE ? ak : dk);
+        pe = and_or(x.myPawns, np, C == WHITE ? dk : ak);
+        myN = andn(myN, x.pinned);
+    }
+
+    // enemy attacks without sliders: king, pawns, knights
+    const u64 eK = side_of<1 - C>(p.kings, W);
+    u64 att;
+    {
+        const u64 pw_ = C == WHITE ? shift<-9>(eP) : shift<7>(eP), pe_ = C == WHITE ? shift<-7>(eP) : shift<9>(eP);
+        att = or3(t.king[ctz(eK) & 63], or_and(andn(pw_, FILEH), pe_, ~FILEA), behind);
+        const u64 h1 = or_and(andn(shift<-1>(eN), FILEH), shift<1>(eN), ~FILEA);
+        const u64 h2 = or_and(andn(shift<-2>(eN), FILEG | FILEH), shift<2>(eN), ~(FILEA | FILEB));
+        att = or3(att, shift<16>(h1), shift<-16>(h1));
+        att = or3(att, shift<8>(h2), shift<-8>(h2));
+    }
+
+    const u64 empty = ~x.occ;
     int n = 0;
-    {
-        const Prop3 e = prop_up<8, ~0ULL>(empty);
-        att |= fill_up_p<8, ~0ULL>(eR, e);
-        n += popc(fill_up_p<8, ~0ULL>(rf, e) & tgt);
+    {   // north and south
+        const Prop3 pn = props<8>(empty), ps = props<-8>(empty);
+        const u64 an = grow<8>(eR, pn), as = grow<-8>(eR, ps);
+        att = or3(att, shift<8>(an), shift<-8>(as));
+        n += popc((grow<8>(rf, pn) << 8) & tgt);
+        n += popc((grow<-8>(rf, ps) >> 8) & tgt);
     }
-    {
-        const Prop3 e = prop_dn<8, ~0ULL>(empty);
-        att |= fill_dn_p<8, ~0ULL>(eR, e);
-        n += popc(fill_dn_p<8, ~0ULL>(rf, e) & tgt);
+    const u64 eA = nor2(x.occ, FILEA), eH = nor2(x.occ, FILEH);
+    {   // east: one add per side
+        att = or_xor(att, eH, east_sum(eR, eH));
+        n += popc(xor_and(eH, east_sum(rr, eH), tgt));
     }
-    att |= east_of(eR, x.occ);
-    n += popc(east_of(rr, x.occ) & tgt);
-    {
-        const u64 occR = rev64(x.occ);
-        att |= rev64(east_of(rev64(eR), occR));
-        n += popc(east_of(rev64(rr), occR) & rev64(tgt));
+    {   // west: east on the bit-reversed board
+        const u64 eoR = rev64(eA);
+        att = or_xor(att, eA, rev64(east_sum(rev64(eR), eoR)));
+        n += popc(xor_and(eA, rev64(east_sum(rev64(rr), eoR)), tgt));
     }
-    {
-        const Prop3 e = prop_up<9, ~FILEA>(empty);
-        att |= fill_up_p<9, ~FILEA>(eB, e);
-        n += popc(fill_up_p<9, ~FILEA>(bd, e) & tgt);
+    {   // north-east and south-west
+        const Prop3 pa = props<9>(eA), pb = props<-9>(eH);
+        att = or_and(att, shift<9>(grow<9>(eB, pa)), ~FILEA);
+        att = or_and(att, shift<-9>(grow<-9>(eB, pb)), ~FILEH);
+        n += popc(and3(shift<9>(grow<9>(bd, pa)), ~FILEA, tgt));
+        n += popc(and3(shift<-9>(grow<-9>(bd, pb)), ~FILEH, tgt));
     }
-    {
-        const Prop3 e = prop_dn<9, ~FILEH>(empty);
-        att |= fill_dn_p<9, ~FILEH>(eB, e);
-        n += popc(fill_dn_p<9, ~FILEH>(bd, e) & tgt);
-    }
-    {
-        const Prop3 e = prop_up<7, ~FILEH>(empty);
-        att |= fill_up_p<7, ~FILEH>(eB, e);
-        n += popc(fill_up_p<7, ~FILEH>(ba, e) & tgt);
-    }
-    {
-        const Prop3 e = prop_dn<7, ~FILEA>(empty);
-        att |= fill_dn_p<7, ~FILEA>(eB, e);
-        n += popc(fill_dn_p<7, ~FILEA>(ba, e) & tgt);
+    {   // north-west and south-east
+        const Prop3 pa = props<7>(eH), pb = props<-7>(eA);
+        att = or_and(att, shift<7>(grow<7>(eB, pa)), ~FILEH);
+        att = or_and(att, shift<-7>(grow<-7>(eB, pb)), ~FILEA);
+        n += popc(and3(shift<7>(grow<7>(ba, pa)), ~FILEH, tgt));
+        n += popc(and3(shift<-7>(grow<-7>(ba, pb)), ~FILEA, tgt));
     }
     x.threatened = att;
-    n += popc(t.king[k] & ~x.mine & ~att);
+    n += popc(and_nor(t.king[k], x.mine, att));
 
-    // pawns
-    const u64 PROMO = RANK1 | RANK8;
-    const u64 p1 = fwd<C>(x.myPawns & (np | fk)) & empty;
+    // pawns: pushes on the low or high half only where the rank allows
+    const u64 PROMO = C == WHITE ? RANK8 : RANK1;
+    const u64 p1 = fwd<C>(pf) & empty;
     const u64 p2 = fwd<C>(p1 & (C == WHITE ? RANK3 : RANK6)) & empty & tgt;
     const u64 t1 = p1 & tgt;
     n += popc(t1) + 3 * popc(t1 & PROMO) + popc(p2);
-    const u64 cw = fwd_west<C>(x.myPawns & (np | (C == WHITE ? ak : dk))) & x.enemy & tgt;
-    const u64 ce = fwd_east<C>(x.myPawns & (np | (C == WHITE ? dk : ak))) & x.enemy & tgt;
+    const u64 et = x.enemy & tgt;
+    const u64 cw = and3(C == WHITE ? shift<7>(pw) : shift<-9>(pw), ~FILEH, et);
+    const u64 ce = and3(C == WHITE ? shift<9>(pe) : shift<-7>(pe), ~FILEA, et);
     n += popc(cw) + 3 * popc(cw & PROMO) + popc(ce) + 3 * popc(ce & PROMO);
-    // knights (pinned knights never move)
-    for (u64 b = p.knights & x.mine & np; b; b &= b - 1) n += popc(t.knight[ctz(b)] & tgt);
+    // knights
+    for (u64 b = myN; b; b &= b - 1) n += popc(t.knight[ctz(b)] & tgt);
     n += castle_count<C>(p, x);
     n += ep_moves<C>(p, t, x, nullptr);
     return (u32)n;
@@ -525,12 +593,12 @@ PG_HD u16 mk_move(int from, int to, int flags) { return (u16)(from | (to << 6) |
 // Pawn captures carry the capture flag; other captures do not, as on the
 // reference's device path (MoveGeneratorBitboard.h:2525-2529).
 template <int C, int V>
-PG_HD u32 generate_moves(const Pos &p, const Tables &t, u16 *out, u32 cap) {
+PG_HD u32 generate_moves(const Pos &p, const Tables &t, u16 *out, u32 cap, u16 *tag = nullptr, u32 owner = 0) {
     MoveCtx x;
     prologue<C, V>(p, t, x);
     const u64 tgt = x.target, empty = ~x.occ, np = ~x.pinned;
-    MoveSink o{out, cap, 0};
-    for (u64 m = king_att(x.myKing) & ~x.mine & ~x.threatened; m; m &= m - 1) o.add(mk_move(x.k, ctz(m), MF_QUIET));
+    MoveSink o{out, cap, 0, tag, owner};
+    for (u64 m = king_att(x.myKing) & ~x.mine & ~x.threatened; m; m &= m - 1) o.add(mk_move(x.k, ctz(m), MF_QUIET), P_KING);
     if (tgt == 0) return o.n;
 
     const u64 fk = file_of(x.k), dk = diag_of(x.k), ak = anti_of(x.k);
@@ -542,15 +610,15 @@ PG_HD u32 generate_moves(const Pos &p, const Tables &t, u16 *out, u32 cap) {
     for (u64 m = p1 & tgt; m; m &= m - 1) {
         const int to = ctz(m), from = to - up;
         if (bit(to) & PROMO) {
-            o.add(mk_move(from, to, MF_PROMO | 0));
-            o.add(mk_move(from, to, MF_PROMO | 1));
-            o.add(mk_move(from, to, MF_PROMO | 3));
-            o.add(mk_move(from, to, MF_PROMO | 2));
+            o.add(mk_move(from, to, MF_PROMO | 0), P_PAWN);
+            o.add(mk_move(from, to, MF_PROMO | 1), P_PAWN);
+            o.add(mk_move(from, to, MF_PROMO | 3), P_PAWN);
+            o.add(mk_move(from, to, MF_PROMO | 2), P_PAWN);
         } else {
-            o.add(mk_move(from, to, MF_QUIET));
+            o.add(mk_move(from, to, MF_QUIET), P_PAWN);
         }
     }
-    for (u64 m = p2; m; m &= m - 1) { const int to = ctz(m); o.add(mk_move(to - 2 * up, to, MF_DOUBLE_PUSH)); }
+    for (u64 m = p2; m; m &= m - 1) { const int to = ctz(m); o.add(mk_move(to - 2 * up, to, MF_DOUBLE_PUSH), P_PAWN); }
     // captures
     const u64 cw = fwd_west<C>(x.myPawns & (np | (C == WHITE ? ak : dk))) & x.enemy & tgt;
     const u64 ce = fwd_east<C>(x.myPawns & (np | (C == WHITE ? dk : ak))) & x.enemy & tgt;
@@ -559,12 +627,12 @@ PG_HD u32 generate_moves(const Pos &p, const Tables &t, u16 *out, u32 cap) {
         for (u64 m = side == 0 ? cw : ce; m; m &= m - 1) {
             const int to = ctz(m), from = to - d;
             if (bit(to) & PROMO) {
-                o.add(mk_move(from, to, MF_PROMO | MF_CAPTURE | 0));
-                o.add(mk_move(from, to, MF_PROMO | MF_CAPTURE | 1));
-                o.add(mk_move(from, to, MF_PROMO | MF_CAPTURE | 3));
-                o.add(mk_move(from, to, MF_PROMO | MF_CAPTURE | 2));
+                o.add(mk_move(from, to, MF_PROMO | MF_CAPTURE | 0), P_PAWN);
+                o.add(mk_move(from, to, MF_PROMO | MF_CAPTURE | 1), P_PAWN);
+                o.add(mk_move(from, to, MF_PROMO | MF_CAPTURE | 3), P_PAWN);
+                o.add(mk_move(from, to, MF_PROMO | MF_CAPTURE | 2), P_PAWN);
             } else {
-                o.add(mk_move(from, to, MF_CAPTURE));
+                o.add(mk_move(from, to, MF_CAPTURE), P_PAWN);
             }
         }
     }
@@ -573,26 +641,28 @@ PG_HD u32 generate_moves(const Pos &p, const Tables &t, u16 *out, u32 cap) {
     if (!x.checkers) {
         const int r = C == WHITE ? pos_wcastle(p) : pos_bcastle(p);
         const int sh = C == WHITE ? 0 : 56;
-        if ((r & 1) && !((0x60ULL << sh) & (x.occ | x.threatened))) o.add(mk_move(4 + sh, 6 + sh, MF_KING_CASTLE));
-        if ((r & 2) && !((0x0EULL << sh) & x.occ) && !((0x0CULL << sh) & x.threatened)) o.add(mk_move(4 + sh, 2 + sh, MF_QUEEN_CASTLE));
+        if ((r & 1) && !((0x60ULL << sh) & (x.occ | x.threatened))) o.add(mk_move(4 + sh, 6 + sh, MF_KING_CASTLE), P_KING);
+        if ((r & 2) && !((0x0EULL << sh) & x.occ) && !((0x0CULL << sh) & x.threatened)) o.add(mk_move(4 + sh, 2 + sh, MF_QUEEN_CASTLE), P_KING);
     }
     // knights
     for (u64 b = p.knights & x.mine & np; b; b &= b - 1) {
         const int s = ctz(b);
-        for (u64 m = t.knight[s] & tgt; m; m &= m - 1) o.add(mk_move(s, ctz(m), MF_QUIET));
+        for (u64 m = t.knight[s] & tgt; m; m &= m - 1) o.add(mk_move(s, ctz(m), MF_QUIET), P_KNIGHT);
     }
     // sliders
     for (u64 b = p.bishopQueens & x.mine; b; b &= b - 1) {
         const int s = ctz(b);
         u64 m = Sl<V>::bishop(t, s, x.occ) & tgt;
         if (x.pinned & bit(s)) m &= t.line[(s << 6) | x.k];
-        for (; m; m &= m - 1) o.add(mk_move(s, ctz(m), MF_QUIET));
+        const int pc = (p.rookQueens & bit(s)) ? P_QUEEN : P_BISHOP;
+        for (; m; m &= m - 1) o.add(mk_move(s, ctz(m), MF_QUIET), pc);
     }
     for (u64 r = p.rookQueens & x.mine; r; r &= r - 1) {
         const int s = ctz(r);
         u64 m = Sl<V>::rook(t, s, x.occ) & tgt;
         if (x.pinned & bit(s)) m &= t.line[(s << 6) | x.k];
-        for (; m; m &= m - 1) o.add(mk_move(s, ctz(m), MF_QUIET));
+        const int pc = (p.bishopQueens & bit(s)) ? P_QUEEN : P_ROOK;
+        for (; m; m &= m - 1) o.add(mk_move(s, ctz(m), MF_QUIET), pc);
     }
     return o.n;
 }
@@ -601,7 +671,6 @@ PG_HD u32 generate_moves(const Pos &p, const Tables &t, u16 *out, u32 cap) {
 // Zobrist word offsets in the 782-word key set (chess.h:326-333)
 PG_HD int zob_piece(int color, int piece /*1..6*/, int sq) { return color * 384 + (piece - 1) * 64 + sq; }
 constexpr int ZOB_CASTLE = 768, ZOB_EP = 772, ZOB_SIDE = 780;
-enum { P_PAWN = 1, P_KNIGHT, P_BISHOP, P_ROOK, P_QUEEN, P_KING };
 
 PG_HD void zob_xor(Key128 &h, const Tables &t, int w) {
     h.lo ^= t.zobLo[w];
@@ -630,12 +699,12 @@ PG_HD int piece_on(const Pos &p, u64 b) {
 // file is recorded only when an enemy pawn can take, and castling rights are
 // cleared by king moves, rook moves from a corner and captures on a corner.
 // With HASH the 128-bit key is updated incrementally.
+// make_move_as takes the moving piece from the caller (the generator records
+// it); make_move looks it up.
 template <int C, bool HASH>
-PG_HD void make_move(Pos &p, u16 mv, Key128 *h, const Tables &t) {
+PG_HD void make_move_as(Pos &p, u16 mv, int piece, Key128 *h, const Tables &t) {
     const int from = mv & 63, to = (mv >> 6) & 63, fl = mv >> 12;
     const u64 src = bit(from), dst = bit(to);
-    int piece = piece_on(p, src);
-    if (!piece) piece = P_PAWN;  // only for a move that is not legal here; keeps key indices in range
     if (HASH) {
         zob_xor(*h, t, zob_piece(C, piece, from));
         const int cap = piece_on(p, dst);
@@ -710,6 +779,13 @@ PG_HD void make_move(Pos &p, u16 mv, Key128 *h, const Tables &t) {
     if (HASH) zob_state(*h, t, p);
 }
 
+template <int C, bool HASH>
+PG_HD void make_move(Pos &p, u16 mv, Key128 *h, const Tables &t) {
+    int piece = piece_on(p, bit(mv & 63));
+    if (!piece) piece = P_PAWN;  // only for a move that is not legal here; keeps key indices in range
+    make_move_as<C, HASH>(p, mv, piece, h, t);
+}
+
 // Full 128-bit key (MoveGeneratorBitboard.h:3914-3980); the side key is
 // present when white is to move.
 PG_HD Key128 zobrist128(const Pos &p, const Tables &t) {
@@ -730,8 +806,9 @@ PG_HD u32 count_any(const Pos &p, const Tables &t) {
     return pos_side(p) ? count_moves<BLACK, V>(p, t) : count_moves<WHITE, V>(p, t);
 }
 template <int V>
-PG_HD u32 generate_any(const Pos &p, const Tables &t, u16 *out, u32 cap = 256) {
-    return pos_side(p) ? generate_moves<BLACK, V>(p, t, out, cap) : generate_moves<WHITE, V>(p, t, out, cap);
+PG_HD u32 generate_any(const Pos &p, const Tables &t, u16 *out, u32 cap = 256, u16 *tag = nullptr, u32 owner = 0) {
+    return pos_side(p) ? generate_moves<BLACK, V>(p, t, out, cap, tag, owner)
+                       : generate_moves<WHITE, V>(p, t, out, cap, tag, owner);
 }
 PG_HD void make_any(Pos &p, u16 mv, const Tables &t) {
     if (pos_side(p)) make_move<BLACK, false>(p, mv, nullptr, t);
