@@ -152,6 +152,89 @@ PG_HD int knight_count(u64 n, u64 t) {
            popc((n >> 10) & ~GH & t) + popc((n >> 6) & ~AB & t);
 }
 
+PG_HD u64 rev64(u64 x) {
+#if defined(__clang__)
+    return __builtin_bitreverse64(x);
+#else
+    x = ((x >> 1) & 0x5555555555555555ULL) | ((x & 0x5555555555555555ULL) << 1);
+    x = ((x >> 2) & 0x3333333333333333ULL) | ((x & 0x3333333333333333ULL) << 2);
+    x = ((x >> 4) & 0x0F0F0F0F0F0F0F0FULL) | ((x & 0x0F0F0F0F0F0F0F0FULL) << 4);
+    return __builtin_bswap64(x);
+#endif
+}
+
+PG_HD u64 nor2(u64 a, u64 b) { return bop3<0x03>(a, b, b); }  // ~(a | b)
+
+// squares east of each slider in s up to and including its first blocker,
+// given eo = ~(occ | FILEH); every slider must be in occ
+PG_HD u64 east_sum(u64 s, u64 eo) { return (andn(s, FILEH) << 1) + eo; }
+
+template <int S> PG_HD u64 shift(u64 x) {
+#if defined(__HIP_DEVICE_COMPILE__)
+    if constexpr (S > 0 && S < 32) {
+        u64 r;
+        asm("v_lshlrev_b64 %0, %1, %2" : "=v"(r) : "i"(S), "v"(x));
+        return r;
+    } else if constexpr (S < 0 && S > -32) {
+        u64 r;
+        asm("v_lshrrev_b64 %0, %1, %2" : "=v"(r) : "i"(-S), "v"(x));
+        return r;
+    }
+#endif
+    return S > 0 ? x << S : x >> -S;
+}
+struct Prop3 {
+    u64 a, b, c;
+};
+// propagators of a direction from the empty squares that may be entered
+template <int S> PG_HD Prop3 props(u64 a) {
+    Prop3 p;
+    p.a = a;
+    p.b = a & shift<S>(a);
+    p.c = p.b & shift<2 * S>(p.b);
+    return p;
+}
+// generators grown through the propagators (the ray squares shifted back
+// by one step, still to be shifted by S and wrap-masked)
+template <int S> PG_HD u64 grow(u64 g, const Prop3 &p) {
+    g = or_and(g, shift<S>(g), p.a);
+    g = or_and(g, shift<2 * S>(g), p.b);
+    if constexpr (S == 8 || S == -8) {
+        // a shift by 32 moves one half into the other: one 32-bit step
+        u32 lo = (u32)g, hi = (u32)(g >> 32);
+        if (S == 8) hi |= lo & (u32)(p.c >> 32);
+        else lo |= hi & (u32)p.c;
+        return ((u64)hi << 32) | lo;
+    }
+    return or_and(g, shift<4 * S>(g), p.c);
+}
+// the side's part of a piece word (every piece word lies inside occ)
+template <int C> PG_HD u64 side_of(u64 b, u64 white) { return C == WHITE ? b & white : andn(b, white); }
+
+// Attacks of the sliders in b (bishop-like) and r (rook-like) through the
+// empty squares e, one pass over the eight directions with the propagators
+// built once per direction; every slider must be off e.
+PG_HD u64 ks_attacks(u64 b, u64 r, u64 e) {
+    const Prop3 pn = props<8>(e), ps = props<-8>(e);
+    u64 a = shift<8>(grow<8>(r, pn)) | shift<-8>(grow<-8>(r, ps));
+    const u64 eA = andn(e, FILEA), eH = andn(e, FILEH);
+    a = or_xor(a, eH, east_sum(r, eH));
+    a = or_xor(a, eA, rev64(east_sum(rev64(r), rev64(eA))));
+    const Prop3 p9 = props<9>(eA), m9 = props<-9>(eH), p7 = props<7>(eH), m7 = props<-7>(eA);
+    a = or_and(a, shift<9>(grow<9>(b, p9)), ~FILEA);
+    a = or_and(a, shift<-9>(grow<-9>(b, m9)), ~FILEH);
+    a = or_and(a, shift<7>(grow<7>(b, p7)), ~FILEH);
+    return or_and(a, shift<-7>(grow<-7>(b, m7)), ~FILEA);
+}
+
+// Attacks of one slider along one line through its square (the line may
+// include the square): (o - s) ^ rev(rev(o) - rev(s)) with o the occupied
+// squares of the line other than s (hyperbola quintessence, no tables).
+PG_HD u64 line_att(u64 sb, u64 occ, u64 line) {
+    const u64 o = bop3<0x40>(occ, line, sb);  // occ & line & ~sb
+    return xor_and(o - sb, rev64(rev64(o) - rev64(sb)), line);
+}
+
 // pawn geometry per side
 template <int C> PG_HD u64 fwd(u64 x) { return C == WHITE ? x << 8 : x >> 8; }
 template <int C> PG_HD u64 fwd_west(u64 x) { return C == WHITE ? (x << 7) & ~FILEH : (x >> 9) & ~FILEH; }
@@ -175,8 +258,14 @@ template <int V> struct Sl;
 
 template <> struct Sl<SL_KOGGE> {
     static constexpr bool kBulk = true;
-    PG_HD static u64 rook(const Tables &, int s, u64 occ) { return ks_rook(bit(s), ~occ); }
-    PG_HD static u64 bishop(const Tables &, int s, u64 occ) { return ks_bishop(bit(s), ~occ); }
+    PG_HD static u64 rook(const Tables &, int s, u64 occ) {
+        const u64 sb = bit(s);
+        return line_att(sb, occ, file_of(s)) | line_att(sb, occ, rank_of(s));
+    }
+    PG_HD static u64 bishop(const Tables &, int s, u64 occ) {
+        const u64 sb = bit(s);
+        return line_att(sb, occ, diag_of(s)) | line_att(sb, occ, anti_of(s));
+    }
 };
 
 // high 32 bits of the low 64-bit word of a*b, i.e. (a*b) >> 32 mod 2^32:
@@ -246,10 +335,9 @@ PG_HD void prologue(const Pos &p, const Tables &t, MoveCtx &x) {
     x.eRooks = eR;
 
     // squares the enemy attacks, sliders looking through our king
-    u64 a = pawn_att<1 - C>(eP) | knight_att(eN) | king_att(eK);
+    u64 a = pawn_att<1 - C>(eP) | knight_att(eN) | t.king[ctz(eK) & 63];
     if (Sl<V>::kBulk) {
-        const u64 e = ~x.occ | x.myKing;
-        a |= ks_bishop(eB, e) | ks_rook(eR, e);
+        a |= ks_attacks(eB, eR, ~x.occ | x.myKing);
     } else {
         const u64 o = x.occ ^ x.myKing;
         for (u64 b = eB; b; b &= b - 1) a |= Sl<V>::bishop(t, ctz(b), o);
@@ -320,8 +408,10 @@ PG_HD int ep_moves(const Pos &p, const Tables &t, const MoveCtx &x, MoveSink *ou
         if (x.pinned & bit(s)) {
             ok = (dst & t.line[(s << 6) | x.k]) != 0;
         } else {
-            const u64 e = ~x.occ | cap | bit(s);
-            ok = ((att_e(x.eRooks, e) | att_w(x.eRooks, e)) & x.myKing) == 0;
+            // both pawns leave the rank: no rook or queen may then see the king along it
+            const u64 e = ~x.occ | cap | bit(s), eH = andn(e, FILEH), eA = andn(e, FILEA);
+            const u64 ray = (eH ^ east_sum(x.eRooks, eH)) | (eA ^ rev64(east_sum(rev64(x.eRooks), rev64(eA))));
+            ok = (ray & x.myKing) == 0;
         }
         if (ok) {
             if (out) out->add((u16)(s | (to << 6) | (MF_EP_CAPTURE << 12)), P_PAWN);
@@ -410,58 +500,6 @@ PG_HD u32 count_moves(const Pos &p, const Tables &t) {
 //    to its first blocker are o ^ (o - 2s) = e ^ (e + 2s); west rays the
 //    same on the bit-reversed board;
 //  * pinned pieces need the king-line masks only when something is pinned.
-PG_HD u64 rev64(u64 x) {
-#if defined(__clang__)
-    return __builtin_bitreverse64(x);
-#else
-    x = ((x >> 1) & 0x5555555555555555ULL) | ((x & 0x5555555555555555ULL) << 1);
-    x = ((x >> 2) & 0x3333333333333333ULL) | ((x & 0x3333333333333333ULL) << 2);
-    x = ((x >> 4) & 0x0F0F0F0F0F0F0F0FULL) | ((x & 0x0F0F0F0F0F0F0F0FULL) << 4);
-    return __builtin_bswap64(x);
-#endif
-}
-
-PG_HD u64 nor2(u64 a, u64 b) { return bop3<0x03>(a, b, b); }  // ~(a | b)
-
-// squares east of each slider in s up to and including its first blocker,
-// given eo = ~(occ | FILEH); every slider must be in occ
-PG_HD u64 east_sum(u64 s, u64 eo) { return (andn(s, FILEH) << 1) + eo; }
-
-template <int S> PG_HD u64 shift(u64 x) {
-#if defined(__HIP_DEVICE_COMPILE__)
-    if constexpr (S > 0 && S < 32) {
-        u64 r;
-        asm("v_lshlrev_b64 %0, %1, %2" : "=v"(r) : "i"(S), "v"(x));
-        return r;
-    } else if constexpr (S < 0 && S > -32) {
-        u64 r;
-        asm("v_lshrrev_b64 %0, %1, %2" : "=v"(r) : "i"(-S), "v"(x));
-        return r;
-    }
-#endif
-    return S > 0 ? x << S : x >> -S;
-}
-struct Prop3 {
-    u64 a, b, c;
-};
-// propagators of a direction from the empty squares that may be entered
-template <int S> PG_HD Prop3 props(u64 a) {
-    Prop3 p;
-    p.a = a;
-    p.b = a & shift<S>(a);
-    p.c = p.b & shift<2 * S>(p.b);
-    return p;
-}
-// generators grown through the propagators (the ray squares shifted back
-// by one step, still to be shifted by S and wrap-masked)
-template <int S> PG_HD u64 grow(u64 g, const Prop3 &p) {
-    g = or_and(g, shift<S>(g), p.a);
-    g = or_and(g, shift<2 * S>(g), p.b);
-    return or_and(g, shift<4 * S>(g), p.c);
-}
-// the side's part of a piece word (every piece word lies inside occ)
-template <int C> PG_HD u64 side_of(u64 b, u64 white) { return C == WHITE ? b & white : andn(b, white); }
-
 template <int C>
 PG_HD u32 count_moves_ks(const Pos &p, const Tables &t) {
     MoveCtx x;
