@@ -30,9 +30,13 @@ sys.path.insert(0, ROOT)
 
 METRIC = "perft leaves/sec on 1 MI355X (exact counts); p50/p99 latency of served queries"
 HBM_PEAK_GBS = 8000.0
-# SQ_INSTS_VALU per leaf item of k_leaf<Kogge-Stone> on the start-position
-# depth-8 leaf level, from the committed rocprofv3 counter profile
-VALU_INSTS_PER_ITEM = None
+# From the rocprofv3 counter runs of this workload with the Kogge-Stone fused
+# leaf kernel (profiles/k_leaf_fused_kogge_start_d7_d8_c2_d6.md):
+#   VALU lane instructions per counted board = SQ_INSTS_VALU * 64 / boards
+#   HBM bytes per parent board = (FETCH_SIZE * 2 + WRITE_SIZE) KiB * 1024 / parents,
+#   the factors from a 1 GiB copy kernel (FETCH_SIZE reports half the bytes read)
+VALU_LANE_INSTS_PER_ITEM = 641
+TRAFFIC_BYTES_PER_PARENT = 52.3
 
 
 def log(*a):
@@ -124,39 +128,53 @@ def leg_tt(pg, dev):
 
 
 def leg_serve(pg, dev):
+    """Served queries: Poisson streams at five rates and one burst, each over
+    its own slice of a pool of distinct seeded queries (no query repeats, so
+    TT hits are genuine transpositions).  The first 512 queries are the
+    golden set; every served result equals the plain one-query-per-launch
+    GPU path, which is also the throughput baseline."""
     import numpy as np
+    from perft_gpu_amd import workloads as W
     from perft_gpu_amd.serve import Server, serve, poisson_arrivals, latency_summary
     c5 = golden("c5.json")
-    pos0 = np.array(c5["words"], np.uint64)
-    dep0 = np.array(c5["depths"], np.uint32)
-    want0 = np.array(c5["perft"], np.uint64)
-    t = time.perf_counter()
-    s = 0
-    for w, d in zip(pos0, dep0):
-        s += dev.perft(pg.pos_from_words(w), int(d), 0)
-    plain = time.perf_counter() - t
-    if s != int(want0.sum()):
+    plan = [(100, 200), (300, 600), (1000, 2000), (3000, 2000), (10000, 2000), (None, 2000)]
+    pool, depths = W.c5_queries(sum(n for _, n in plan))
+    if not np.array_equal(pool[:len(c5["words"])], np.array(c5["words"], np.uint64)):
+        raise SystemExit("served-query generator differs from the golden set")
+    want = np.zeros(len(pool), np.uint64)
+    took = np.zeros(len(pool))
+    for i in range(len(pool)):
+        t = time.perf_counter()
+        want[i] = dev.perft(pg.pos_from_words(pool[i]), int(depths[i]), 0)
+        took[i] = time.perf_counter() - t
+    plain = float(took.sum())
+    if [int(x) for x in want[:len(c5["perft"])]] != c5["perft"]:
         raise SystemExit("MISMATCH in plain served-query loop")
-    rates = []
+    log("plain one-query-per-launch: %d queries %.2f s" % (len(pool), plain))
+    rates, first = [], 0
     with pg.TranspositionTable(dev, 8 << 30) as tt, Server(dev, slots=4, slot_bytes=16 << 30, tt=tt) as srv:
-        for rate in (100, 300, 1000, 3000, 10000):
-            n = int(min(2000, max(200, rate * 2)))
-            idx = np.arange(n) % len(pos0)
-            arr = poisson_arrivals(n, float(rate), seed=rate)
-            r = serve(srv, pos0[idx], dep0[idx], arr)
-            if not np.array_equal(r["results"], want0[idx]):
-                raise SystemExit("MISMATCH in served results at rate %d" % rate)
-            leaves = int(want0[idx].sum())
-            row = {"rate_per_s": rate, "queries": n, "elapsed_s": r["elapsed"], "leaves_per_s": leaves / r["elapsed"],
-                   "batches": r["batches"], "mean_batch": float(np.mean(r["batch_sizes"])),
-                   "max_busy_slots": r["max_busy_slots"]}
+        for rate, n in plan:
+            idx = np.arange(first, first + n)
+            first += n
+            arr = np.zeros(n) if rate is None else poisson_arrivals(n, float(rate), seed=rate)
+            r = serve(srv, pool[idx], depths[idx], arr)
+            if not np.array_equal(r["results"], want[idx]):
+                raise SystemExit("MISMATCH in served results at rate %s" % rate)
+            leaves = int(want[idx].sum())
+            row = {"rate_per_s": rate if rate else "burst", "queries": n, "elapsed_s": r["elapsed"],
+                   "leaves_per_s": leaves / r["elapsed"], "batches": r["batches"],
+                   "mean_batch": float(np.mean(r["batch_sizes"])), "max_busy_slots": r["max_busy_slots"]}
+            row["plain_leaves_per_s"] = leaves / float(took[idx].sum())
             row.update(latency_summary(r["latency"]))
             rates.append(row)
-            log("serve rate %d: p50 %.2f ms p99 %.2f ms %.1f G leaves/s" % (rate, row["p50_ms"], row["p99_ms"],
+            log("serve rate %s: p50 %.2f ms p99 %.2f ms %.1f G leaves/s" % (rate, row["p50_ms"], row["p99_ms"],
                                                                           row["leaves_per_s"] / 1e9))
-    return {"workload": "512 seeded queries (ply 0-60; depth 3/4/5/6/7 at 40/30/20/8/2%), cycled; Poisson arrivals; "
-                        "4 slots, shared 8 GB TT", "rates": rates,
-            "plain_sequential": {"queries": len(pos0), "s": plain, "leaves_per_s": int(want0.sum()) / plain},
+    burst = rates[-1]
+    return {"workload": "%d distinct seeded queries (ply 0-60; depth 3/4/5/6/7 at 40/30/20/8/2%%), one slice per "
+                        "rate; Poisson arrivals at 100-10,000/s, then a burst of 2000 at once; 4 slots, shared 8 GB TT"
+                        % len(pool), "rates": rates,
+            "plain_one_query_per_launch": {"queries": len(pool), "s": plain, "leaves_per_s": int(want.sum()) / plain},
+            "burst_vs_plain": burst["leaves_per_s"] / burst["plain_leaves_per_s"],
             "exact": True}
 
 
@@ -248,17 +266,23 @@ def main():
         bytes_per_launch = (6 * acc["items"] + 48 * acc["parents"]) / max(1, acc["launches"])
         kname, bnote = "k_leaf", "6 B per (parent, move) item + 48 B per parent board"
     achieved = bytes_per_launch / (avg_ms * 1e-3) / 1e9 if avg_ms > 0 else 0.0
+    traffic = None
+    if fused and best == pg.SLIDER_KOGGE_STONE:
+        traffic = TRAFFIC_BYTES_PER_PARENT * acc["parents"] / max(1, acc["launches"])
     roofline = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
                 "kernel": "%s<%s>" % (kname, pg.SLIDER_NAMES[best]), "launches": acc["launches"],
                 "avg_launch_ms": avg_ms, "items_per_launch": acc["items"] / max(1, acc["launches"]),
                 "bytes_per_launch": bytes_per_launch,
                 "note": "integer VALU-bound kernel; algorithmic bytes: " + bnote}
     items_per_s = acc["items"] / (acc["ms"] * 1e-3) if acc["ms"] > 0 else 0.0
-    valu = {"bound": "valu", "items_per_s": items_per_s, "peak_lane_ops": 256 * 64 * 2.4e9}
-    if VALU_INSTS_PER_ITEM:
-        valu["achieved_lane_ops"] = items_per_s * VALU_INSTS_PER_ITEM * 64
-        valu["frac"] = valu["achieved_lane_ops"] / valu["peak_lane_ops"]
+    valu = {"bound": "valu", "unit": "int32 lane-ops/s", "peak": 256 * 64 * 2.4e9, "items_per_s": items_per_s,
+            "note": "the kernel's real bound: 256 CUs x 64 lanes x 2.4 GHz; counted boards per second from the "
+                    "HIP-event time above"}
+    if best == pg.SLIDER_KOGGE_STONE:
+        valu["lane_insts_per_item"] = VALU_LANE_INSTS_PER_ITEM
+        valu["achieved"] = items_per_s * VALU_LANE_INSTS_PER_ITEM
+        valu["frac"] = valu["achieved"] / valu["peak"]
 
     legs = {}
     for leg in [x for x in args.legs.split(",") if x and x != "none"]:
