@@ -85,3 +85,36 @@ def test_special_rule_positions(dev, pg, v):
         p = pg.pos_from_words(item["words"])
         for d, want in enumerate(item["perft"]):
             assert dev.perft(p, d, v) == want, (item["fen"], d, v)
+
+
+# positions without a legal move: checkmate (fool's mate, white to move) and
+# stalemate (black to move)
+TERMINAL = ["rnb1kbnr/pppp1ppp/8/4p3/6Pq/5P2/PPPPP2P/RNBQKBNR w KQkq - 1 3",
+            "7k/5Q2/6K1/8/8/8/8/8 b - - 0 1"]
+
+
+def test_terminal_roots_on_every_path(dev, pg, orc):
+    import numpy as np
+    boards = [pg.read_fen_string(f) for f in TERMINAL]
+    for b in boards:
+        assert orc.count_moves(pg.positions_array([b])[0].tolist()) == 0
+    for d in range(0, 5):
+        want = 1 if d == 0 else 0
+        for b in boards:
+            print("terminal plain/tt d%d" % d, flush=True)
+            assert dev.perft(b, d, 0) == want
+    with pg.TranspositionTable(dev, 1 << 30) as tt:
+        for d in range(0, 5):
+            for b in boards:
+                assert tt.perft(b, d) == (1 if d == 0 else 0)
+    # terminal roots inside a batch of ordinary ones, and an empty batch
+    items = golden("random.json")[:64]
+    arr = np.concatenate([pg.positions_array(boards), np.array([it["words"] for it in items], np.uint64)])
+    for dedup in (False, True):
+        print("terminal batch dedup %d" % dedup, flush=True)
+        got, _ = dev.perft_batch(arr, 3, 0, dedup)
+        assert [int(x) for x in got] == [0, 0] + [it["perft"][3] for it in items]
+        got, _ = dev.perft_batch(pg.positions_array(boards), 3, 0, dedup)
+        assert [int(x) for x in got] == [0, 0]
+    got, _ = dev.perft_batch(np.zeros((0, 6), np.uint64), 3, 0, True)
+    assert len(got) == 0
