@@ -335,7 +335,15 @@ PG_HD void prologue(const Pos &p, const Tables &t, MoveCtx &x) {
     x.eRooks = eR;
 
     // squares the enemy attacks, sliders looking through our king
-    u64 a = pawn_att<1 - C>(eP) | knight_att(eN) | t.king[ctz(eK) & 63];
+    u64 a;
+    {
+        const u64 pw = C == WHITE ? shift<-9>(eP) : shift<7>(eP), pe = C == WHITE ? shift<-7>(eP) : shift<9>(eP);
+        a = or3(t.king[ctz(eK) & 63], andn(pw, FILEH), andn(pe, FILEA));
+        const u64 h1 = or_and(andn(shift<-1>(eN), FILEH), shift<1>(eN), ~FILEA);
+        const u64 h2 = or_and(andn(shift<-2>(eN), FILEG | FILEH), shift<2>(eN), ~(FILEA | FILEB));
+        a = or3(a, shift<16>(h1), shift<-16>(h1));
+        a = or3(a, shift<8>(h2), shift<-8>(h2));
+    }
     if (Sl<V>::kBulk) {
         a |= ks_attacks(eB, eR, ~x.occ | x.myKing);
     } else {
@@ -440,46 +448,46 @@ PG_HD u32 count_moves(const Pos &p, const Tables &t) {
     if constexpr (V == SL_KOGGE) return count_moves_ks<C>(p, t);
     MoveCtx x;
     prologue<C, V>(p, t, x);
-    const u64 tgt = x.target, empty = ~x.occ, np = ~x.pinned;
-    const u64 PROMO = RANK1 | RANK8;
-    int n = popc(king_att(x.myKing) & ~x.mine & ~x.threatened);
+    const u64 tgt = x.target, empty = ~x.occ;
+    int n = popc(and_nor(t.king[x.k], x.mine, x.threatened));
     if (tgt == 0) return n;  // double check: king moves only
 
-    const u64 fk = file_of(x.k), rk = rank_of(x.k), dk = diag_of(x.k), ak = anti_of(x.k);
+    // movers; a pinned piece moves only along its pin line
+    u64 pf = x.myPawns, pw = x.myPawns, pe = x.myPawns, myN = p.knights & x.mine;
+    if (x.pinned) {
+        const u64 np = ~x.pinned, dk = diag_of(x.k), ak = anti_of(x.k);
+        pf = and_or(x.myPawns, np, file_of(x.k));
+        pw = and_or(x.myPawns, np, C == WHITE ? ak : dk);
+        pe = and_or(x.myPawns, np, C == WHITE ? dk : ak);
+        myN = andn(myN, x.pinned);
+    }
 
     // pawns
-    const u64 p1 = fwd<C>(x.myPawns & (np | fk)) & empty;
+    const u64 PROMO = C == WHITE ? RANK8 : RANK1;
+    const u64 p1 = fwd<C>(pf) & empty;
     const u64 p2 = fwd<C>(p1 & (C == WHITE ? RANK3 : RANK6)) & empty & tgt;
     const u64 t1 = p1 & tgt;
     n += popc(t1) + 3 * popc(t1 & PROMO) + popc(p2);
-    const u64 cw = fwd_west<C>(x.myPawns & (np | (C == WHITE ? ak : dk))) & x.enemy & tgt;
-    const u64 ce = fwd_east<C>(x.myPawns & (np | (C == WHITE ? dk : ak))) & x.enemy & tgt;
+    const u64 et = x.enemy & tgt;
+    const u64 cw = and3(C == WHITE ? shift<7>(pw) : shift<-9>(pw), ~FILEH, et);
+    const u64 ce = and3(C == WHITE ? shift<9>(pe) : shift<-7>(pe), ~FILEA, et);
     n += popc(cw) + 3 * popc(cw & PROMO) + popc(ce) + 3 * popc(ce & PROMO);
 
     // knights (pinned knights never move)
-    n += knight_count(p.knights & x.mine & np, tgt);
+    for (u64 b = myN; b; b &= b - 1) n += popc(t.knight[ctz(b)] & tgt);
 
-    // sliders
-    const u64 rq = p.rookQueens & x.mine, bq = p.bishopQueens & x.mine;
-    if (Sl<V>::kBulk) {
-        const u64 rf = rq & (np | fk), rr = rq & (np | rk), bd = bq & (np | dk), ba = bq & (np | ak);
-        n += popc(att_n(rf, empty) & tgt) + popc(att_s(rf, empty) & tgt) +
-             popc(att_e(rr, empty) & tgt) + popc(att_w(rr, empty) & tgt) +
-             popc(att_ne(bd, empty) & tgt) + popc(att_sw(bd, empty) & tgt) +
-             popc(att_nw(ba, empty) & tgt) + popc(att_se(ba, empty) & tgt);
-    } else {
-        for (u64 b = bq; b; b &= b - 1) {
-            const int s = ctz(b);
-            u64 m = Sl<V>::bishop(t, s, x.occ) & tgt;
-            if (x.pinned & bit(s)) m &= t.line[(s << 6) | x.k];
-            n += popc(m);
-        }
-        for (u64 r = rq; r; r &= r - 1) {
-            const int s = ctz(r);
-            u64 m = Sl<V>::rook(t, s, x.occ) & tgt;
-            if (x.pinned & bit(s)) m &= t.line[(s << 6) | x.k];
-            n += popc(m);
-        }
+    // sliders, per piece
+    for (u64 b = p.bishopQueens & x.mine; b; b &= b - 1) {
+        const int s = ctz(b);
+        u64 m = Sl<V>::bishop(t, s, x.occ) & tgt;
+        if (x.pinned & bit(s)) m &= t.line[(s << 6) | x.k];
+        n += popc(m);
+    }
+    for (u64 r = p.rookQueens & x.mine; r; r &= r - 1) {
+        const int s = ctz(r);
+        u64 m = Sl<V>::rook(t, s, x.occ) & tgt;
+        if (x.pinned & bit(s)) m &= t.line[(s << 6) | x.k];
+        n += popc(m);
     }
     n += castle_count<C>(p, x);
     n += ep_moves<C>(p, t, x, nullptr);
