@@ -166,6 +166,9 @@ def leg_serve(pg, dev):
                    "mean_batch": float(np.mean(r["batch_sizes"])), "max_busy_slots": r["max_busy_slots"]}
             row["plain_leaves_per_s"] = leaves / float(took[idx].sum())
             row.update(latency_summary(r["latency"]))
+            lat, dq = np.asarray(r["latency"]), depths[idx]
+            row["by_depth"] = {int(d): dict(latency_summary(lat[dq == d]), queries=int((dq == d).sum()))
+                               for d in sorted(set(int(x) for x in dq))}
             rates.append(row)
             log("serve rate %s: p50 %.2f ms p99 %.2f ms %.1f G leaves/s" % (rate, row["p50_ms"], row["p99_ms"],
                                                                           row["leaves_per_s"] / 1e9))
