@@ -21,7 +21,7 @@ CPW = {
 # published CPW perft values (chessprogramming.org "Perft Results")
 CPW_COUNTS = {
     "start": [1, 20, 400, 8902, 197281, 4865609, 119060324, 3195901860, 84998978956, 2439530234167,
-              69352859712417],
+              69352859712417, 2097651003696806],
     "kiwipete": [1, 48, 2039, 97862, 4085603, 193690690, 8031647685],
     "pos3": [1, 14, 191, 2812, 43238, 674624, 11030083, 178633661],
     "pos4": [1, 6, 264, 9467, 422333, 15833292, 706045033],
