@@ -105,11 +105,17 @@ def poisson_arrivals(n, rate, seed=0):
     return np.cumsum(rng.exponential(1.0 / rate, size=n))
 
 
-def serve(server, positions, depths, arrivals, max_batch=1024):
+def serve(server, positions, depths, arrivals, max_batch=1024, long_depth=6, long_slots=None):
     """Runs the request loop until every query is answered.
 
     positions: (n, 6) uint64; depths: (n,); arrivals: seconds from start.
+    Batches of depth >= long_depth may hold at most long_slots slots at once
+    (default: half of them), so short queries never queue behind a slot-full
+    of long ones.  Among the depths that may start, the one whose oldest
+    query arrived first goes next.
     Returns dict with per-query results, latencies (s) and counters."""
+    if long_slots is None:
+        long_slots = max(1, server.slots // 2)
     positions = positions_array(positions)
     n = len(positions)
     results = np.zeros(n, dtype=np.uint64)
@@ -122,6 +128,7 @@ def serve(server, positions, depths, arrivals, max_batch=1024):
     overlap_samples = 0
     max_busy = 0
     batch_sizes = []
+    running = {}   # slot -> depth of its batch
     t0 = time.perf_counter()
     while finished < n:
         now = time.perf_counter() - t0
@@ -133,6 +140,7 @@ def serve(server, positions, depths, arrivals, max_batch=1024):
             if r is None:
                 continue
             counts, st, ids = r
+            running.pop(s, None)
             tdone = time.perf_counter() - t0
             for q, c in zip(ids, counts):
                 results[q] = c
@@ -140,7 +148,8 @@ def serve(server, positions, depths, arrivals, max_batch=1024):
                 latency[q] = tdone - arrivals[q]
             finished += len(ids)
         for s in server.idle_slots():
-            live = [d for d in waiting if waiting[d]]
+            nlong = sum(1 for d in running.values() if d >= long_depth)
+            live = [d for d in waiting if waiting[d] and (d < long_depth or nlong < long_slots)]
             if not live:
                 break
             d = min(live, key=lambda k: arrivals[waiting[k][0]])
@@ -152,6 +161,7 @@ def serve(server, positions, depths, arrivals, max_batch=1024):
                     break
                 ids.append(waiting[d].popleft())
             server.submit(s, positions[ids], d, ids)
+            running[s] = d
             batches += 1
             batch_sizes.append(len(ids))
         busy = len(server.busy_slots())
