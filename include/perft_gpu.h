@@ -140,7 +140,8 @@ typedef struct pg_tt pg_tt;
 
 /* setupHashTables128b / allocAndClearMem (launcher.h:155-320): one HBM
  * allocation of at most `bytes`, split into one table per remaining depth
- * (16-byte entries for depths 2-4, 32-byte entries above), zeroed. */
+ * (16-byte entries for depths 3-4, 32-byte entries above; none for depth 2),
+ * zeroed. */
 int pg_tt_create(pg_ctx *ctx, uint64_t bytes, pg_tt **tt);
 int pg_tt_clear(pg_tt *tt);
 /* freeHashTables (launcher.h:322) */

@@ -195,9 +195,13 @@ int Engine::perft_tt(TTable &tt, const Pos &root, u32 depth, int v, u64 *out) {
 struct pg_tt : public pg::TTable {};
 struct pg_ctx;  // == pg::Engine (engine.hip)
 
-// Splits `bytes` over remaining depths 2..15: half to depth 2, a quarter to
-// depth 3, an eighth to depth 4 (16-byte entries), the last eighth shared by
-// depths 5..15 (32-byte entries); each table a power of two of slots.
+// Splits `bytes` over remaining depths 3..15: a quarter to depth 3, a quarter
+// to depth 4 (16-byte entries), the other half shared by depths 5..15 (32-byte
+// entries); each table a power of two of slots.  Remaining depth 2 gets no
+// table: the in-level duplicate collapse already removes the transpositions
+// within a level, and a hit there saves only one perft(2), less than its
+// probe and store cost (start-position perft(11) with 64 GB: 24.1 s without
+// it, 36.0 s with half the budget on it).
 static void tt_plan(u64 bytes, int bits[TT_MAX_DEPTH], u64 sizes[TT_MAX_DEPTH]) {
     for (int r = 0; r < TT_MAX_DEPTH; r++) { bits[r] = 0; sizes[r] = 0; }
     auto fit = [](u64 budget, u64 entry) {
@@ -205,12 +209,12 @@ static void tt_plan(u64 bytes, int bits[TT_MAX_DEPTH], u64 sizes[TT_MAX_DEPTH]) 
         while (b < 40 && (entry << (b + 1)) <= budget) b++;
         return b;
     };
-    const u64 share[5] = {0, 0, bytes / 2, bytes / 4, bytes / 8};
-    for (int r = 2; r <= TT_SHALLOW_MAX; r++) {
+    const u64 share[5] = {0, 0, 0, bytes / 4, bytes / 4};
+    for (int r = 3; r <= TT_SHALLOW_MAX; r++) {
         bits[r] = fit(share[r], 16);
         sizes[r] = 16ull << bits[r];
     }
-    const u64 deep = (bytes / 8) / (TT_MAX_DEPTH - 1 - TT_SHALLOW_MAX);
+    const u64 deep = (bytes / 2) / (TT_MAX_DEPTH - 1 - TT_SHALLOW_MAX);
     for (int r = TT_SHALLOW_MAX + 1; r < TT_MAX_DEPTH; r++) {
         bits[r] = fit(deep, 32);
         sizes[r] = 32ull << bits[r];

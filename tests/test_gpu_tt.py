@@ -20,7 +20,7 @@ def tt(dev, pg):
 def test_tt_layout(tt):
     nbytes, bits = tt.info()
     assert nbytes <= 2 << 30
-    assert bits[2] > bits[3] > bits[4] > 0 and all(b > 0 for b in bits[5:])
+    assert bits[2] == -1 and bits[3] == bits[4] > 0 and all(b > 0 for b in bits[5:])
 
 
 @pytest.mark.parametrize("v", [0, 2, 3])
