@@ -144,13 +144,6 @@ PG_HD u64 knight_att(u64 n) {
     u64 h1 = l1 | r1, h2 = l2 | r2;
     return (h1 << 16) | (h1 >> 16) | (h2 << 8) | (h2 >> 8);
 }
-// sum over all knights of their moves landing in t
-PG_HD int knight_count(u64 n, u64 t) {
-    const u64 AB = FILEA | FILEB, GH = FILEG | FILEH;
-    return popc((n << 17) & ~FILEA & t) + popc((n << 15) & ~FILEH & t) + popc((n << 10) & ~AB & t) +
-           popc((n << 6) & ~GH & t) + popc((n >> 17) & ~FILEH & t) + popc((n >> 15) & ~FILEA & t) +
-           popc((n >> 10) & ~GH & t) + popc((n >> 6) & ~AB & t);
-}
 
 PG_HD u64 rev64(u64 x) {
 #if defined(__clang__)
@@ -362,7 +355,7 @@ PG_HD void prologue(const Pos &p, const Tables &t, MoveCtx &x) {
         if (b == 0) chk |= bit(s);
         else if (!several(b)) pinned |= b;
     }
-    chk |= (knight_att(x.myKing) & eN) | (pawn_att<C>(x.myKing) & eP);
+    chk |= (t.knight[x.k] & eN) | (pawn_att<C>(x.myKing) & eP);
     x.pinned = pinned & x.mine;
     x.checkers = chk;
     u64 tgt = ~x.mine;
