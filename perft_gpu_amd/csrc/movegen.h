@@ -627,6 +627,14 @@ PG_HD u32 count_moves_ks(const Pos &p, const Tables &t) {
 // ---------------------------------------------------------------- generate
 PG_HD u16 mk_move(int from, int to, int flags) { return (u16)(from | (to << 6) | (flags << 12)); }
 
+// calls f(square) for every set square of m, low half then high half (a
+// 32-bit find-first-set and clear per square instead of the 64-bit ones)
+template <class F>
+PG_HD void each_square(u64 m, F &&f) {
+    for (u32 lo = (u32)m; lo; lo &= lo - 1) f(__builtin_ctz(lo));
+    for (u32 hi = (u32)(m >> 32); hi; hi &= hi - 1) f(32 + __builtin_ctz(hi));
+}
+
 // Writes the legal moves of p to out (at most cap of them); returns the
 // number of legal moves (at most 218).
 // Pawn captures carry the capture flag; other captures do not, as on the
@@ -635,46 +643,43 @@ template <int C, int V>
 PG_HD u32 generate_moves(const Pos &p, const Tables &t, u16 *out, u32 cap, u16 *tag = nullptr, u32 owner = 0) {
     MoveCtx x;
     prologue<C, V>(p, t, x);
-    const u64 tgt = x.target, empty = ~x.occ, np = ~x.pinned;
+    const u64 tgt = x.target, empty = ~x.occ;
     MoveSink o{out, cap, 0, tag, owner};
-    for (u64 m = king_att(x.myKing) & ~x.mine & ~x.threatened; m; m &= m - 1) o.add(mk_move(x.k, ctz(m), MF_QUIET), P_KING);
+    each_square(and_nor(t.king[x.k], x.mine, x.threatened),
+                [&](int to) { o.add(mk_move(x.k, to, MF_QUIET), P_KING); });
     if (tgt == 0) return o.n;
 
-    const u64 fk = file_of(x.k), dk = diag_of(x.k), ak = anti_of(x.k);
+    // a pinned pawn moves only along its pin line
+    u64 pf = x.myPawns, pw = x.myPawns, pe = x.myPawns;
+    if (x.pinned) {
+        const u64 np = ~x.pinned, dk = diag_of(x.k), ak = anti_of(x.k);
+        pf = and_or(x.myPawns, np, file_of(x.k));
+        pw = and_or(x.myPawns, np, C == WHITE ? ak : dk);
+        pe = and_or(x.myPawns, np, C == WHITE ? dk : ak);
+    }
     const int up = C == WHITE ? 8 : -8;
-    const u64 PROMO = RANK1 | RANK8;
+    const u64 PROMO = C == WHITE ? RANK8 : RANK1;
+    auto promos = [&](int from, int to, int cap) {
+        o.add(mk_move(from, to, MF_PROMO | cap | 0), P_PAWN);
+        o.add(mk_move(from, to, MF_PROMO | cap | 1), P_PAWN);
+        o.add(mk_move(from, to, MF_PROMO | cap | 3), P_PAWN);
+        o.add(mk_move(from, to, MF_PROMO | cap | 2), P_PAWN);
+    };
     // pushes
-    const u64 p1 = fwd<C>(x.myPawns & (np | fk)) & empty;
+    const u64 p1 = fwd<C>(pf) & empty;
     const u64 p2 = fwd<C>(p1 & (C == WHITE ? RANK3 : RANK6)) & empty & tgt;
-    for (u64 m = p1 & tgt; m; m &= m - 1) {
-        const int to = ctz(m), from = to - up;
-        if (bit(to) & PROMO) {
-            o.add(mk_move(from, to, MF_PROMO | 0), P_PAWN);
-            o.add(mk_move(from, to, MF_PROMO | 1), P_PAWN);
-            o.add(mk_move(from, to, MF_PROMO | 3), P_PAWN);
-            o.add(mk_move(from, to, MF_PROMO | 2), P_PAWN);
-        } else {
-            o.add(mk_move(from, to, MF_QUIET), P_PAWN);
-        }
-    }
-    for (u64 m = p2; m; m &= m - 1) { const int to = ctz(m); o.add(mk_move(to - 2 * up, to, MF_DOUBLE_PUSH), P_PAWN); }
+    each_square(andn(p1 & tgt, PROMO), [&](int to) { o.add(mk_move(to - up, to, MF_QUIET), P_PAWN); });
+    each_square(p1 & tgt & PROMO, [&](int to) { promos(to - up, to, 0); });
+    each_square(p2, [&](int to) { o.add(mk_move(to - 2 * up, to, MF_DOUBLE_PUSH), P_PAWN); });
     // captures
-    const u64 cw = fwd_west<C>(x.myPawns & (np | (C == WHITE ? ak : dk))) & x.enemy & tgt;
-    const u64 ce = fwd_east<C>(x.myPawns & (np | (C == WHITE ? dk : ak))) & x.enemy & tgt;
-    for (int side = 0; side < 2; side++) {
-        const int d = side == 0 ? (C == WHITE ? 7 : -9) : (C == WHITE ? 9 : -7);
-        for (u64 m = side == 0 ? cw : ce; m; m &= m - 1) {
-            const int to = ctz(m), from = to - d;
-            if (bit(to) & PROMO) {
-                o.add(mk_move(from, to, MF_PROMO | MF_CAPTURE | 0), P_PAWN);
-                o.add(mk_move(from, to, MF_PROMO | MF_CAPTURE | 1), P_PAWN);
-                o.add(mk_move(from, to, MF_PROMO | MF_CAPTURE | 3), P_PAWN);
-                o.add(mk_move(from, to, MF_PROMO | MF_CAPTURE | 2), P_PAWN);
-            } else {
-                o.add(mk_move(from, to, MF_CAPTURE), P_PAWN);
-            }
-        }
-    }
+    const u64 et = x.enemy & tgt;
+    const u64 cw = and3(C == WHITE ? shift<7>(pw) : shift<-9>(pw), ~FILEH, et);
+    const u64 ce = and3(C == WHITE ? shift<9>(pe) : shift<-7>(pe), ~FILEA, et);
+    constexpr int dw = C == WHITE ? 7 : -9, de = C == WHITE ? 9 : -7;
+    each_square(andn(cw, PROMO), [&](int to) { o.add(mk_move(to - dw, to, MF_CAPTURE), P_PAWN); });
+    each_square(andn(ce, PROMO), [&](int to) { o.add(mk_move(to - de, to, MF_CAPTURE), P_PAWN); });
+    each_square(cw & PROMO, [&](int to) { promos(to - dw, to, MF_CAPTURE); });
+    each_square(ce & PROMO, [&](int to) { promos(to - de, to, MF_CAPTURE); });
     ep_moves<C>(p, t, x, &o);
     // castling
     if (!x.checkers) {
@@ -683,10 +688,10 @@ PG_HD u32 generate_moves(const Pos &p, const Tables &t, u16 *out, u32 cap, u16 *
         if ((r & 1) && !((0x60ULL << sh) & (x.occ | x.threatened))) o.add(mk_move(4 + sh, 6 + sh, MF_KING_CASTLE), P_KING);
         if ((r & 2) && !((0x0EULL << sh) & x.occ) && !((0x0CULL << sh) & x.threatened)) o.add(mk_move(4 + sh, 2 + sh, MF_QUEEN_CASTLE), P_KING);
     }
-    // knights
-    for (u64 b = p.knights & x.mine & np; b; b &= b - 1) {
+    // knights (pinned knights never move)
+    for (u64 b = andn(p.knights & x.mine, x.pinned); b; b &= b - 1) {
         const int s = ctz(b);
-        for (u64 m = t.knight[s] & tgt; m; m &= m - 1) o.add(mk_move(s, ctz(m), MF_QUIET), P_KNIGHT);
+        each_square(t.knight[s] & tgt, [&](int to) { o.add(mk_move(s, to, MF_QUIET), P_KNIGHT); });
     }
     // sliders
     for (u64 b = p.bishopQueens & x.mine; b; b &= b - 1) {
@@ -694,14 +699,14 @@ PG_HD u32 generate_moves(const Pos &p, const Tables &t, u16 *out, u32 cap, u16 *
         u64 m = Sl<V>::bishop(t, s, x.occ) & tgt;
         if (x.pinned & bit(s)) m &= t.line[(s << 6) | x.k];
         const int pc = (p.rookQueens & bit(s)) ? P_QUEEN : P_BISHOP;
-        for (; m; m &= m - 1) o.add(mk_move(s, ctz(m), MF_QUIET), pc);
+        each_square(m, [&](int to) { o.add(mk_move(s, to, MF_QUIET), pc); });
     }
     for (u64 r = p.rookQueens & x.mine; r; r &= r - 1) {
         const int s = ctz(r);
         u64 m = Sl<V>::rook(t, s, x.occ) & tgt;
         if (x.pinned & bit(s)) m &= t.line[(s << 6) | x.k];
         const int pc = (p.bishopQueens & bit(s)) ? P_QUEEN : P_ROOK;
-        for (; m; m &= m - 1) o.add(mk_move(s, ctz(m), MF_QUIET), pc);
+        each_square(m, [&](int to) { o.add(mk_move(s, to, MF_QUIET), pc); });
     }
     return o.n;
 }
