@@ -35,7 +35,7 @@ HBM_PEAK_GBS = 8000.0
 #   VALU lane instructions per counted board = SQ_INSTS_VALU * 64 / boards
 #   HBM bytes per parent board = (FETCH_SIZE * 2 + WRITE_SIZE) KiB * 1024 / parents,
 #   the factors from a 1 GiB copy kernel (FETCH_SIZE reports half the bytes read)
-VALU_LANE_INSTS_PER_ITEM = 641
+VALU_LANE_INSTS_PER_ITEM = 631
 TRAFFIC_BYTES_PER_PARENT = 52.3
 
 
