@@ -51,3 +51,33 @@ def test_served_batch_overflow_finishes_in_halves(dev, pg):
         counts, st, _ = r
     assert [int(x) for x in counts] == [x["perft"][5] for x in g]
     assert st["splits"] > 0
+
+
+def test_server_rejects_misuse(dev, pg):
+    """Host-side argument checks of pg_server_submit and of the perft entry
+    points: nothing invalid reaches a kernel, and the slot stays usable."""
+    from perft_gpu_amd.serve import Server
+    g = golden("c2.json")["positions"]
+    arr = np.array([x["words"] for x in g], np.uint64)
+    bad = np.zeros((1, 6), np.uint64)          # no kings
+    with Server(dev, slots=2, slot_bytes=64 << 20, max_roots=8, tt=None) as srv:
+        with pytest.raises(pg.PerftError):
+            srv.submit(2, arr[:1], 3)           # no such slot
+        with pytest.raises(pg.PerftError):
+            srv.submit(0, arr[:9], 3)           # more roots than the slot holds
+        with pytest.raises(pg.PerftError):
+            srv.submit(0, bad, 3)
+        srv.submit(0, arr[:4], 3, list(range(4)))
+        with pytest.raises(pg.PerftError):
+            srv.submit(0, arr[:4], 3)           # slot still busy
+        r = None
+        while r is None:
+            r = srv.poll(0)
+        assert [int(x) for x in r[0]] == [x["perft"][3] for x in g[:4]]
+    with pytest.raises(pg.PerftError):
+        dev.perft_batch(bad, 3, 0, False)
+    with pytest.raises(pg.PerftError):
+        dev.perft(pg.pos_from_words(bad[0]), 3, 0)
+    with pg.TranspositionTable(dev, 1 << 30) as tt:
+        with pytest.raises(pg.PerftError):
+            tt.perft(pg.pos_from_words(bad[0]), 3)
