@@ -144,6 +144,13 @@ typedef struct pg_tt pg_tt;
  * zeroed. */
 int pg_tt_create(pg_ctx *ctx, uint64_t bytes, pg_tt **tt);
 int pg_tt_clear(pg_tt *tt);
+/* Checkpoint and resume of a table's contents (the reference keeps its
+ * transpositions across runs in the disk hash perfthash.dat, launcher.h:700-789
+ * and :1711): pg_tt_save writes a header with the table layout and a checksum,
+ * then the entries; pg_tt_load accepts only a file written from a table of the
+ * same size and layout, and clears the table if the data do not match. */
+int pg_tt_save(pg_tt *tt, const char *path);
+int pg_tt_load(pg_tt *tt, const char *path);
 /* freeHashTables (launcher.h:322) */
 void pg_tt_destroy(pg_tt *tt);
 /* allocated bytes and log2(slots) per remaining depth (-1: no table) */

@@ -130,6 +130,8 @@ def lib():
         "pg_tt_info": (ctypes.c_int, [ctypes.c_void_p, P(u64), ctypes.c_void_p]),
         "pg_tt_counters": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p]),
         "pg_tt_poison": (ctypes.c_int, [ctypes.c_void_p, u64]),
+        "pg_tt_save": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_char_p]),
+        "pg_tt_load": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_char_p]),
         "pg_perft_tt": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, P(Pos48), ctypes.c_uint32, ctypes.c_int,
                                        P(u64)]),
         "pg_set_dedup": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int]),
@@ -364,6 +366,14 @@ class TranspositionTable:
 
     def poison(self, seed=1):
         _check(lib().pg_tt_poison(self._h, seed), "pg_tt_poison")
+
+    def save(self, path):
+        """pg_tt_save: write the table's entries (with layout and checksum) to path."""
+        _check(lib().pg_tt_save(self._h, os.fsencode(path)), "pg_tt_save")
+
+    def load(self, path):
+        """pg_tt_load: restore entries saved from a table of the same size and layout."""
+        _check(lib().pg_tt_load(self._h, os.fsencode(path)), "pg_tt_load")
 
     def info(self):
         b = u64()

@@ -11,6 +11,7 @@ transposition-table path runs (the reference's default build,
 switches.h:39).
 """
 import argparse
+import os
 import sys
 import time
 
@@ -27,6 +28,8 @@ def main(argv=None):
     ap.add_argument("--slider", type=int, default=pg.SLIDER_AUTO)
     ap.add_argument("--min-depth", type=int, default=3)
     ap.add_argument("--divide", action="store_true", help="print perft(depth-1) per root move at the last depth")
+    ap.add_argument("--tt-file", default=None,
+                    help="with --tt: resume the table from this checkpoint if it exists, and save it at the end")
     args = ap.parse_args(argv)
     if args.fen is None or len(args.fen) <= 5:
         print("\nUsage perft_gpu_amd.cli <fen> <depth> [<numGPUs>] [<launchdepth>]")
@@ -41,6 +44,9 @@ def main(argv=None):
     print(pg.pos_to_fen(pos))
     with pg.Device(0) as dev:
         tt = pg.TranspositionTable(dev, int(args.tt * (1 << 30))) if args.tt > 0 else None
+        if tt and args.tt_file and os.path.exists(args.tt_file):
+            tt.load(args.tt_file)
+            print("resumed the transposition table from %s" % args.tt_file)
         try:
             for d in range(min(args.min_depth, args.depth), args.depth + 1):
                 t0 = time.perf_counter()
@@ -53,6 +59,9 @@ def main(argv=None):
                 for m, c in div.items():
                     print("%s: %d" % (pg.move_to_str(m), c))
                 print("total: %d" % sum(div.values()))
+            if tt and args.tt_file:
+                tt.save(args.tt_file)
+                print("saved the transposition table to %s" % args.tt_file)
         finally:
             if tt:
                 tt.close()

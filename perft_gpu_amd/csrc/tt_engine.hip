@@ -9,8 +9,10 @@
 // summed into its parent (calcPerftNFromPerftNminus1, perft_bb.h:849-961).
 // A level that does not fit the arena is split in halves; the first half's
 // stores then serve the second half.
-#include <cstring>
+#include <algorithm>
 #include <chrono>
+#include <cstdio>
+#include <cstring>
 #include "engine.h"
 
 using namespace pg;
@@ -309,6 +311,99 @@ int pg_perft_tt(pg_ctx *ctx, pg_tt *t, const pg_pos48 *pos, uint32_t depth, int 
     }
     (void)hipSetDevice(e->device);
     return e->perft_tt(*t, p, depth, v, reinterpret_cast<u64 *>(out));
+}
+
+// Checkpoint of a table's contents: a 128-byte header (magic, bytes, the
+// per-depth layout, a wrapping 64-bit sum of all words) followed by the raw
+// entries, copied through one pinned staging buffer.  Entries verify their own
+// key on every probe, so a table restored from any file can only miss, never
+// return a wrong count; the sum still rejects a damaged or foreign file.
+struct TTFileHeader {
+    char magic[8];
+    u64 bytes;
+    int32_t bits[TT_MAX_DEPTH];
+    u64 sum;
+    u8 pad[128 - 8 - 8 - 4 * TT_MAX_DEPTH - 8];
+};
+static_assert(sizeof(TTFileHeader) == 128, "checkpoint header is 128 bytes");
+static const char TT_MAGIC[8] = {'P', 'G', 'T', 'T', '0', '0', '0', '1'};
+constexpr size_t TT_STAGE = 64ull << 20;
+
+static int tt_copy_file(pg_tt *t, FILE *f, bool save, u64 *sum) {
+    u8 *stage = nullptr;
+    if (hipHostMalloc(&stage, TT_STAGE) != hipSuccess) return PG_ENOMEM;
+    int rc = PG_OK;
+    u64 s = 0;
+    for (u64 o = 0; o < t->bytes && rc == PG_OK; o += TT_STAGE) {
+        const size_t n = (size_t)std::min<u64>(TT_STAGE, t->bytes - o);
+        if (save) {
+            if (hipMemcpy(stage, t->mem + o, n, hipMemcpyDeviceToHost) != hipSuccess) rc = PG_EHIP;
+            else if (fwrite(stage, 1, n, f) != n) rc = PG_EINVAL;
+        } else {
+            if (fread(stage, 1, n, f) != n) rc = PG_EINVAL;
+            else if (hipMemcpy(t->mem + o, stage, n, hipMemcpyHostToDevice) != hipSuccess) rc = PG_EHIP;
+        }
+        const u64 *w = reinterpret_cast<const u64 *>(stage);
+        for (size_t i = 0; i < n / 8; i++) s += w[i];
+    }
+    (void)hipHostFree(stage);
+    *sum = s;
+    return rc;
+}
+
+int pg_tt_save(pg_tt *t, const char *path) {
+    if (!t || !path) return PG_EINVAL;
+    (void)hipSetDevice(t->eng->device);
+    if (hipStreamSynchronize(t->eng->stream) != hipSuccess) return PG_EHIP;
+    FILE *f = fopen(path, "wb");
+    if (!f) {
+        set_error("cannot write %s", path);
+        return PG_EINVAL;
+    }
+    TTFileHeader h{};
+    memcpy(h.magic, TT_MAGIC, 8);
+    h.bytes = t->bytes;
+    for (int r = 0; r < TT_MAX_DEPTH; r++) h.bits[r] = t->view.base[r] ? t->bits[r] : -1;
+    int rc = fwrite(&h, sizeof(h), 1, f) == 1 ? PG_OK : PG_EINVAL;
+    u64 sum = 0;
+    if (rc == PG_OK) rc = tt_copy_file(t, f, true, &sum);
+    if (rc == PG_OK) {
+        h.sum = sum;
+        if (fseek(f, 0, SEEK_SET) != 0 || fwrite(&h, sizeof(h), 1, f) != 1) rc = PG_EINVAL;
+    }
+    if (fclose(f) != 0 && rc == PG_OK) rc = PG_EINVAL;
+    if (rc) set_error("writing %s failed", path);
+    return rc;
+}
+
+int pg_tt_load(pg_tt *t, const char *path) {
+    if (!t || !path) return PG_EINVAL;
+    (void)hipSetDevice(t->eng->device);
+    if (hipStreamSynchronize(t->eng->stream) != hipSuccess) return PG_EHIP;
+    FILE *f = fopen(path, "rb");
+    if (!f) {
+        set_error("cannot read %s", path);
+        return PG_EINVAL;
+    }
+    TTFileHeader h{};
+    int rc = fread(&h, sizeof(h), 1, f) == 1 ? PG_OK : PG_EINVAL;
+    if (rc == PG_OK && (memcmp(h.magic, TT_MAGIC, 8) != 0 || h.bytes != t->bytes)) rc = PG_EINVAL;
+    for (int r = 0; rc == PG_OK && r < TT_MAX_DEPTH; r++)
+        if (h.bits[r] != (t->view.base[r] ? t->bits[r] : -1)) rc = PG_EINVAL;
+    if (rc) {
+        fclose(f);
+        set_error("%s is not a checkpoint of a table with this layout", path);
+        return rc;
+    }
+    u64 sum = 0;
+    rc = tt_copy_file(t, f, false, &sum);
+    fclose(f);
+    if (rc == PG_OK && sum != h.sum) rc = PG_EINVAL;
+    if (rc) {
+        set_error("reading %s failed or its checksum does not match; the table is cleared", path);
+        (void)hipMemset(t->mem, 0, t->bytes);
+    }
+    return rc;
 }
 
 int pg_tt_poison(pg_tt *t, uint64_t seed) {

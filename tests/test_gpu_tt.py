@@ -86,3 +86,30 @@ def test_tt_depth9_equals_no_tt_path(dev, pg):
     print("no-tt d9", flush=True)
     b = dev.perft(W.CPW["start"], 9, 0)
     assert a == b == W.CPW_COUNTS["start"][9]
+
+
+def test_tt_checkpoint_resume(dev, pg, tmp_path):
+    """pg_tt_save / pg_tt_load: a restored table answers the root from the
+    checkpoint; a file of another layout or with damaged data is refused and
+    leaves a cleared, still exact table."""
+    path = str(tmp_path / "tt.ckpt")
+    with pg.TranspositionTable(dev, 1 << 30) as a:
+        assert a.perft(W.CPW["start"], 6) == W.CPW_COUNTS["start"][6]
+        a.save(path)
+    with pg.TranspositionTable(dev, 1 << 30) as b:
+        b.load(path)
+        assert b.perft(W.CPW["start"], 6) == W.CPW_COUNTS["start"][6]
+        # answered by the root entry: no level below was searched
+        assert int(b.counters().sum()) == 0 and dev.last_stats()["leaf_launches"] == 0
+    with pg.TranspositionTable(dev, 2 << 30) as other:
+        with pytest.raises(pg.PerftError):
+            other.load(path)
+    with open(path, "r+b") as f:                      # damage one entry word
+        f.seek(128 + 4096)
+        w = f.read(8)
+        f.seek(128 + 4096)
+        f.write(bytes(x ^ 0xFF for x in w))
+    with pg.TranspositionTable(dev, 1 << 30) as d:
+        with pytest.raises(pg.PerftError):
+            d.load(path)
+        assert d.perft(W.CPW["start"], 6) == W.CPW_COUNTS["start"][6]
