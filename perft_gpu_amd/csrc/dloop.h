@@ -290,6 +290,7 @@ template <int V>
 __global__ __launch_bounds__(LEAF_WG) __attribute__((amdgpu_waves_per_eu(V == SL_KOGGE ? 4 : 5, 8))) void kd_leaf(Plan *P, int l, u32 *err, Tables t) {
     const DLevel L = P->lv[l];
     __shared__ Pos sb[LEAF_WG];
+    constexpr int LEAF_CAP = leaf_cap<true>();
     __shared__ u16 smv[LEAF_CAP];
     __shared__ u16 stag[LEAF_CAP];  // owner | moving piece << 8
     __shared__ u32 sper[LEAF_WG];

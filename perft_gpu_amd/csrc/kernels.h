@@ -187,16 +187,22 @@ __device__ __forceinline__ u64 block_excl_scan(u64 v, u64 *total) {
 // ---------------------------------------------------------------- fused leaf
 // The last two plies without touching HBM for the (parent, move) items: a
 // work-group takes 256 boards of ply D-2 and their move counts, scans the
-// counts in LDS, generates the moves into an LDS window of LEAF_CAP slots
+// counts in LDS, generates the moves into an LDS window of leaf_cap slots
 // (boards that do not fit wait for the next window) and counts the legal
 // moves of every child.  PER = also write each board's perft(2).
-constexpr int LEAF_WG = 256, LEAF_CAP = 6144;
+// The window is as large as four work-groups per CU allow (160 KB of LDS:
+// at most 40 KB each with the 12 KB of boards, 4 bytes per slot, and 1 KB of
+// per-board sums when PER): 256 boards average about 7000 children, so most
+// tiles then take one pass.
+constexpr int LEAF_WG = 256;
+template <bool PER> constexpr int leaf_cap() { return PER ? 6848 : 7040; }
 
 template <int V, bool PER>
 __global__ __launch_bounds__(LEAF_WG) __attribute__((amdgpu_waves_per_eu(V == SL_KOGGE ? 4 : 5, 8))) void k_leaf_fused(const Pos *__restrict__ boards, const u32 *__restrict__ cnt, u64 n,
                                                         u64 *acc, u64 *items, u64 *__restrict__ per, u32 *err, Tables t) {
     static_assert(V != SL_BYTE, "the byte-lookup tables leave no LDS for the move window");
     __shared__ Pos sb[LEAF_WG];
+    constexpr int LEAF_CAP = leaf_cap<PER>();
     __shared__ u16 smv[LEAF_CAP];
     __shared__ u16 stag[LEAF_CAP];  // owner | moving piece << 8
     __shared__ u32 sper[PER ? LEAF_WG : 1];
