@@ -52,6 +52,7 @@ struct Engine {
 
     void *alloc(size_t bytes);
     u32 grid(u64 n, int wg) const;
+    u32 leaf_grid(u64 n) const;  // work-groups for the fused leaf
     int read_u64(const u64 *d, u64 *h);
     int scan(const u32 *cnt, u64 n, u64 *off, u64 *total);
     int leaf(int v, const Pos *b, u64 n, const u32 *par, const u16 *mv, u64 m);

@@ -93,7 +93,7 @@ static void enqueue_search(const Engine &e, hipStream_t s, Plan *P, const GraphS
     if (D == 1 || v == SL_BYTE) {
         kd_perft1<<<gWork, 256, 0, s>>>(P, last < 0 ? 0 : last);
     } else {
-        PG_VARIANT_CALL(v, cap_leaf, s, P, last, gWork, e.dt);
+        PG_VARIANT_CALL(v, cap_leaf, s, P, last, e.cus * LEAF_BLOCKS_PER_CU, e.dt);
     }
     for (int l = lastExpand; l >= 0; l--) {
         kd_resolve<<<gWork, 256, 0, s>>>(P, l, tt, D - l - 1, ctr);

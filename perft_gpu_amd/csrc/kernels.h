@@ -194,7 +194,7 @@ __device__ __forceinline__ u64 block_excl_scan(u64 v, u64 *total) {
 // at most 40 KB each with the 12 KB of boards, 4 bytes per slot, and 1 KB of
 // per-board sums when PER): 256 boards average about 7000 children, so most
 // tiles then take one pass.
-constexpr int LEAF_WG = 256;
+constexpr int LEAF_WG = 256, LEAF_BLOCKS_PER_CU = 64;
 template <bool PER> constexpr int leaf_cap() { return PER ? 6848 : 7040; }
 
 template <int V, bool PER>

@@ -27,7 +27,7 @@ template <int V> static void launch_expand_tt(Engine &c, TTable &tt, const Pos *
 }
 template <int V> static void launch_leaf_per(Engine &c, const Pos *B, const u32 *cnt, u64 n, u64 *P) {
     if constexpr (V != SL_BYTE)
-        k_leaf_fused<V, true><<<c.grid(n, LEAF_WG), LEAF_WG, 0, c.stream>>>(B, cnt, n, c.dAcc, c.dItems, P, c.dErr, c.dt);
+        k_leaf_fused<V, true><<<c.leaf_grid(n), LEAF_WG, 0, c.stream>>>(B, cnt, n, c.dAcc, c.dItems, P, c.dErr, c.dt);
 }
 template <int V> static void launch_gen(Engine &c, const Pos *b, u64 n, const u64 *off, u16 *mv, u32 *par) {
     constexpr int W = wg_size<V>();
