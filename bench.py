@@ -35,7 +35,7 @@ HBM_PEAK_GBS = 8000.0
 #   VALU lane instructions per counted board = SQ_INSTS_VALU * 64 / boards
 #   HBM bytes per parent board = (FETCH_SIZE * 2 + WRITE_SIZE) KiB * 1024 / parents,
 #   the factors from a 1 GiB copy kernel (FETCH_SIZE reports half the bytes read)
-VALU_LANE_INSTS_PER_ITEM = 631
+VALU_LANE_INSTS_PER_ITEM = 629
 TRAFFIC_BYTES_PER_PARENT = 52.3
 
 
@@ -279,12 +279,17 @@ def main():
                 "bytes_per_launch": bytes_per_launch,
                 "note": "integer VALU-bound kernel; algorithmic bytes: " + bnote}
     items_per_s = acc["items"] / (acc["ms"] * 1e-3) if acc["ms"] > 0 else 0.0
-    valu = {"bound": "valu", "unit": "int32 lane-ops/s", "peak": 256 * 64 * 2.4e9, "items_per_s": items_per_s,
-            "note": "the kernel's real bound: 256 CUs x 64 lanes x 2.4 GHz; counted boards per second from the "
-                    "HIP-event time above"}
+    # VALU issue: gfx950 issues the full-rate (VOP3) class -- bitop3, 64-bit
+    # shifts, popcount -- at one wave instruction per CU per clock and simple
+    # two-operand logic/move ops at up to two (a micro-benchmark measured 0.95
+    # and 1.6-1.7); the leaf's mix is mostly the former
+    valu = {"bound": "valu", "unit": "wave instructions per CU per clock (256 CUs, 2.4 GHz)", "peak": 2.0,
+            "single_issue_rate": 1.0, "items_per_s": items_per_s,
+            "note": "the kernel's real bound; counted boards per second from the HIP-event time above, "
+                    "instructions per board from the rocprofv3 counter profile"}
     if best == pg.SLIDER_KOGGE_STONE:
         valu["lane_insts_per_item"] = VALU_LANE_INSTS_PER_ITEM
-        valu["achieved"] = items_per_s * VALU_LANE_INSTS_PER_ITEM
+        valu["achieved"] = items_per_s * VALU_LANE_INSTS_PER_ITEM / 64 / (256 * 2.4e9)
         valu["frac"] = valu["achieved"] / valu["peak"]
 
     legs = {}
