@@ -134,33 +134,50 @@ PG_HD bool same_board(const Pos &a, const Pos &b) {
 // the set becomes its owner and the child is resolved from it later.  The
 // set has at least twice as many slots as children, so a probe sequence
 // always meets an empty slot; the loop is also capped at the set size.
+// one child after its first probe: follow the probe sequence until the child
+// owns a slot or finds its equal
+__device__ __forceinline__ void dedup_finish(u64 i, bool live, u64 s, u32 cur, Pos *__restrict__ children,
+                                             const Key128 *__restrict__ ckeys, u32 *__restrict__ cnt,
+                                             u8 *__restrict__ state, u32 *__restrict__ dup, u32 *set, u64 setMask,
+                                             u32 *err, TTCounters *ctr, int rc) {
+    bool isDup = false;
+    if (live) {
+        for (u64 step = 0;; step++) {
+            if (cur == 0) break;  // claimed: this child owns the position
+            const u32 o = cur - 1;
+            if (ckeys[o].lo == ckeys[i].lo && ckeys[o].hi == ckeys[i].hi && same_board(children[o], children[i])) {
+                dup[i] = o;
+                isDup = true;
+                break;
+            }
+            if (step >= setMask) { *err = 3; break; }
+            s = (s + 1) & setMask;
+            cur = atomicCAS(&set[s], 0u, (u32)i + 1);
+        }
+    }
+    if (ctr) wave_count(&ctr->c[rc][3], isDup);
+    if (isDup) {
+        state[i] = ST_DUP;
+        cnt[i] = 0;
+        children[i].white = 0;
+    }
+}
+
+// Each thread takes two children per step and issues both first probes
+// before waiting on either, so twice as many random accesses are in flight.
 static __global__ __launch_bounds__(256) void k_dedup(Pos *__restrict__ children, const Key128 *__restrict__ ckeys, u64 m,
                                                u32 *__restrict__ cnt, u8 *__restrict__ state, u32 *__restrict__ dup,
                                                u32 *set, u64 setMask, u32 *err, TTCounters *ctr, int rc) {
-    PG_GRID_LOOP(i, m) {
-        bool isDup = false;
-        if (state[i] == ST_LIVE) {
-            const Pos me = children[i];
-            u64 s = (ckeys[i].lo >> 7) & setMask;
-            for (u64 step = 0;; step++) {
-                if (step > setMask) { *err = 3; break; }
-                u32 cur = atomicCAS(&set[s], 0u, (u32)i + 1);
-                if (cur == 0) break;  // claimed: this child owns the position
-                const u32 o = cur - 1;
-                if (ckeys[o].lo == ckeys[i].lo && ckeys[o].hi == ckeys[i].hi && same_board(children[o], me)) {
-                    dup[i] = o;
-                    isDup = true;
-                    break;
-                }
-                s = (s + 1) & setMask;
-            }
-        }
-        if (ctr) wave_count(&ctr->c[rc][3], isDup);
-        if (isDup) {
-            state[i] = ST_DUP;
-            cnt[i] = 0;
-            children[i].white = 0;
-        }
+    const u64 stride = (u64)gridDim.x * blockDim.x;
+    for (u64 i0 = (u64)blockIdx.x * blockDim.x + threadIdx.x; i0 < m; i0 += 2 * stride) {
+        const u64 i1 = i0 + stride;
+        const bool has1 = i1 < m;
+        const bool l0 = state[i0] == ST_LIVE, l1 = has1 && state[i1] == ST_LIVE;
+        const u64 s0 = (ckeys[i0].lo >> 7) & setMask, s1 = l1 ? (ckeys[i1].lo >> 7) & setMask : 0;
+        const u32 c0 = l0 ? atomicCAS(&set[s0], 0u, (u32)i0 + 1) : 0;
+        const u32 c1 = l1 ? atomicCAS(&set[s1], 0u, (u32)i1 + 1) : 0;
+        dedup_finish(i0, l0, s0, c0, children, ckeys, cnt, state, dup, set, setMask, err, ctr, rc);
+        if (has1) dedup_finish(i1, l1, s1, c1, children, ckeys, cnt, state, dup, set, setMask, err, ctr, rc);
     }
 }
 
