@@ -255,33 +255,48 @@ static __global__ __launch_bounds__(256) void kd_clear_set(Plan *P, int l) {
     PG_GRID_LOOP(i, L.setMask + 1) L.set[i] = 0;
 }
 
+// one child after its first probe (kd_dedup): follow the probe sequence
+// until the child owns a slot or finds its equal
+__device__ __forceinline__ void kd_dedup_finish(Plan *P, const DLevel &L, const DLevel &C, u64 i, bool live, u64 s,
+                                                u32 cur, TTCounters *ctr, int rc) {
+    bool isDup = false;
+    if (live) {
+        for (u64 step = 0;; step++) {
+            if (cur == 0) break;
+            const u32 o = cur - 1;
+            if (C.k[o].lo == C.k[i].lo && C.k[o].hi == C.k[i].hi && same_board(C.b[o], C.b[i])) {
+                C.dup[i] = o;
+                isDup = true;
+                break;
+            }
+            if (step >= L.setMask) { P->err = 3; break; }
+            s = (s + 1) & L.setMask;
+            cur = atomicCAS(&L.set[s], 0u, (u32)i + 1);
+        }
+    }
+    if (ctr) wave_count(&ctr->c[rc][3], isDup);
+    if (isDup) {
+        C.state[i] = ST_DUP;
+        C.cnt[i] = 0;
+    }
+}
+
+// in-level duplicates of level l+1 (as k_dedup: two children per thread,
+// both first probes in flight)
 static __global__ __launch_bounds__(256) void kd_dedup(Plan *P, int l, TTCounters *ctr, int rc) {
     const DLevel L = P->lv[l];
     const DLevel C = P->lv[l + 1];
     if (!L.set) return;
-    PG_GRID_LOOP(i, C.n) {
-        bool isDup = false;
-        if (C.state[i] == ST_LIVE) {
-            const Pos me = C.b[i];
-            u64 s = (C.k[i].lo >> 7) & L.setMask;
-            for (u64 step = 0;; step++) {
-                if (step > L.setMask) { P->err = 3; break; }
-                const u32 cur = atomicCAS(&L.set[s], 0u, (u32)i + 1);
-                if (cur == 0) break;
-                const u32 o = cur - 1;
-                if (C.k[o].lo == C.k[i].lo && C.k[o].hi == C.k[i].hi && same_board(C.b[o], me)) {
-                    C.dup[i] = o;
-                    isDup = true;
-                    break;
-                }
-                s = (s + 1) & L.setMask;
-            }
-        }
-        if (ctr) wave_count(&ctr->c[rc][3], isDup);
-        if (isDup) {
-            C.state[i] = ST_DUP;
-            C.cnt[i] = 0;
-        }
+    const u64 stride = (u64)gridDim.x * blockDim.x;
+    for (u64 i0 = (u64)blockIdx.x * blockDim.x + threadIdx.x; i0 < C.n; i0 += 2 * stride) {
+        const u64 i1 = i0 + stride;
+        const bool has1 = i1 < C.n;
+        const bool l0 = C.state[i0] == ST_LIVE, l1 = has1 && C.state[i1] == ST_LIVE;
+        const u64 s0 = (C.k[i0].lo >> 7) & L.setMask, s1 = l1 ? (C.k[i1].lo >> 7) & L.setMask : 0;
+        const u32 c0 = l0 ? atomicCAS(&L.set[s0], 0u, (u32)i0 + 1) : 0;
+        const u32 c1 = l1 ? atomicCAS(&L.set[s1], 0u, (u32)i1 + 1) : 0;
+        kd_dedup_finish(P, L, C, i0, l0, s0, c0, ctr, rc);
+        if (has1) kd_dedup_finish(P, L, C, i1, l1, s1, c1, ctr, rc);
     }
 }
 
