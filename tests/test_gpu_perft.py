@@ -118,3 +118,19 @@ def test_terminal_roots_on_every_path(dev, pg, orc):
         assert [int(x) for x in got] == [0, 0]
     got, _ = dev.perft_batch(np.zeros((0, 6), np.uint64), 3, 0, True)
     assert len(got) == 0
+
+
+def test_c_caller_of_the_abi(pg):
+    """perft_gpu_amd/perft_c (plain C, linked against the library) prints the
+    reference's lines with the published counts, with and without a TT."""
+    import os
+    import subprocess
+    exe = os.path.join(os.path.dirname(pg.LIB_PATH), "perft_c")
+    assert os.path.exists(exe), "perft_c is built with the library (make -C perft_gpu_amd)"
+    out = subprocess.run([exe, W.CPW["start"], "6"], capture_output=True, text=True, timeout=120)
+    assert out.returncode == 0, out.stderr
+    for d in range(1, 7):
+        assert "GPU Perft %d: %d," % (d, W.CPW_COUNTS["start"][d]) in out.stdout
+    out = subprocess.run([exe, W.CPW["kiwipete"], "4", "1"], capture_output=True, text=True, timeout=120)
+    assert out.returncode == 0, out.stderr
+    assert "GPU Perft 4: %d," % W.CPW_COUNTS["kiwipete"][4] in out.stdout
