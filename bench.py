@@ -7,22 +7,34 @@ positions (start position + 10..60 uniform-random legal plies) to depth 6 --
 107.2e9 leaves per step, every count checked against the reference's values.
 
 Prints one JSON line: leaves/s over the timed steps, per-variant timings,
-the leaf kernel's roofline figures (HIP-event timed launches) and the CPU
-baseline (the reference's own CPU perft compiled from its sources, timed on
-this host on a bounded sample).  Further legs, reported beside the headline:
-  batch  4096 seeded random positions at depth 5 in one graph launch
-         (autotuned), against the one-position-per-launch loop;
-  tt     transposition-table perft of the start position at depth 9 and 10
-         with 16 GB and 64 GB tables;
-  serve  served queries (512 seeded positions, depths 3-7 drawn 40/30/20/8/2%)
-         at Poisson rates 100..10000/s: leaves/s and p50/p99 latency.
+the leaf kernel's roofline (integer VALU issue against the chip's vector
+peak, HBM bytes beside it) and the CPU baseline (the reference's own CPU
+perft compiled from its sources, timed on this host on a bounded sample).
+Legs beside the headline (--legs, comma list):
+  counters   two rocprofv3 --pmc passes over one step of this workload plus
+             the calib leg in a child process: VALU instructions per counted
+             board, HBM bytes per launch (scaled by the copy kernel's
+             counters), clock cycles -- measured by this run, not pasted
+  calib      1 GiB copy bandwidth and the VALU issue-rate probes
+  batch      4096 seeded random positions at depth 5 in one graph launch
+             (autotuned), against the one-position-per-launch loop
+  tt         transposition-table perft of the start position at depth 9 and
+             10 with 16 GB and 64 GB tables, with per-depth table counters
+  serve      served queries (distinct seeded positions, depths 3-7 drawn
+             40/30/20/8/2%) at Poisson rates 100..10000/s and a burst: leaves/s
+             and p50/p99 latency, with and without a shared table
+  leafmodes  start perft(7)/(8) per slider variant, fused vs materialized leaf
 """
 import argparse
+import csv
+import glob
 import json
 import os
 import platform
+import shutil
 import subprocess
 import sys
+import tempfile
 import time
 
 ROOT = os.path.dirname(os.path.abspath(__file__))
@@ -30,13 +42,14 @@ sys.path.insert(0, ROOT)
 
 METRIC = "perft leaves/sec on 1 MI355X (exact counts); p50/p99 latency of served queries"
 HBM_PEAK_GBS = 8000.0
-# From the rocprofv3 counter runs of this workload with the Kogge-Stone fused
-# leaf kernel (profiles/k_leaf_fused_kogge_start_d7_d8_c2_d6.md):
-#   VALU lane instructions per counted board = SQ_INSTS_VALU * 64 / boards
-#   HBM bytes per parent board = (FETCH_SIZE * 2 + WRITE_SIZE) KiB * 1024 / parents,
-#   the factors from a 1 GiB copy kernel (FETCH_SIZE reports half the bytes read)
-VALU_LANE_INSTS_PER_ITEM = 629
-TRAFFIC_BYTES_PER_PARENT = 52.3
+# Used only when the counters leg cannot run (no rocprofv3): VALU wave
+# instructions per counted board of k_leaf_fused<kogge_stone> measured by
+# rocprofv3 on this workload (profiles/k_leaf_fused_kogge_start_d7_d8_c2_d6.md)
+PROFILE_VALU_INSTS_PER_BOARD = 629
+PROFILE_SOURCE = "profiles/k_leaf_fused_kogge_start_d7_d8_c2_d6.md"
+LEAF_KERNEL = "k_leaf_fused<0, false>"
+COPY_BYTES = 1 << 30
+COPY_ITERS = 3
 
 
 def log(*a):
@@ -110,21 +123,159 @@ def leg_batch(pg, dev):
             "speedup_vs_plain_loop": loop / dt, "exact": True}
 
 
-def leg_tt(pg, dev):
+def leg_tt(pg, dev, sizes=(16, 64), depths=(9, 10)):
     from perft_gpu_amd import workloads as W
     out = []
-    for gb in (16, 64):
+    for gb in sizes:
         with pg.TranspositionTable(dev, gb << 30) as tt:
-            for d in (9, 10):
+            for d in depths:
                 t = time.perf_counter()
                 n = tt.perft(pg.START_FEN, d, 0)
                 dt = time.perf_counter() - t
                 if n != W.CPW_COUNTS["start"][d]:
                     raise SystemExit("MISMATCH TT perft(%d)" % d)
-                out.append({"tt_gb": gb, "depth": d, "perft": n, "s": dt, "effective_leaves_per_s": n / dt})
+                st = dev.last_stats()
+                out.append({"tt_gb": gb, "depth": d, "perft": n, "s": dt, "effective_leaves_per_s": n / dt,
+                            "peak_arena_gb": st["peak_arena"] / 1e9})
                 log("tt %d GB d%d %.3f s" % (gb, d, dt))
-    return {"workload": "start position, TT perft(9) and perft(10), tables cleared per size", "runs": out,
+            c = tt.counters()
+            out[-1]["counters_by_remaining_depth"] = {
+                int(r): {"probes": int(c[r][0]), "hits": int(c[r][1]), "stores": int(c[r][2]),
+                         "in_level_duplicates": int(c[r][3])} for r in range(2, max(depths) + 1) if c[r].any()}
+    return {"workload": "start position, TT perft(%s) with %s GB tables, each table cleared at creation and "
+                        "filled by the depths before" % ("/".join(map(str, depths)), "/".join(map(str, sizes))),
+            "runs": out, "exact": True}
+
+
+def leg_calib(pg, dev):
+    """Copy bandwidth of a 1 GiB copy and the sustained issue rate of four
+    VALU instruction classes (wave instructions per CU per clock)."""
+    ms, gbs = dev.copy_bandwidth(COPY_BYTES, COPY_ITERS)
+    probes = {}
+    for kind, name in dev.ISSUE_KINDS.items():
+        rate, pms, n, clk, cus = dev.issue_rate(kind, 4096)
+        probes[name] = {"wave_insts_per_cu_per_clock": rate, "ms": pms, "wave_insts": n}
+        log("issue probe %-24s %.3f wave instructions per CU per clock" % (name, rate))
+    log("copy %d MiB: %.3f ms, %.0f GB/s" % (COPY_BYTES >> 20, ms, gbs))
+    return {"copy": {"bytes": COPY_BYTES, "iters": COPY_ITERS, "ms_per_launch": ms, "gb_per_s_read_plus_write": gbs},
+            "issue_probes": probes, "clock_mhz": clk / 1e3, "cus": cus}
+
+
+def leg_leafmodes(pg, dev):
+    """Start perft(7)/(8) per slider variant, fused last two plies (leaf_mode
+    0) against the materialized (parent, move) item list (leaf_mode 1)."""
+    from perft_gpu_amd import workloads as W
+    rows = []
+    for mode in (0, 1):
+        with pg.Device(0, arena_bytes=48 << 30, timing=True, leaf_mode=mode) as d:
+            for v in range(4):
+                for depth in (7, 8):
+                    d.perft(pg.START_FEN, depth, v)   # warm
+                    t = time.perf_counter()
+                    n = d.perft(pg.START_FEN, depth, v)
+                    dt = time.perf_counter() - t
+                    if n != W.CPW_COUNTS["start"][depth]:
+                        raise SystemExit("MISMATCH leaf mode %d variant %d d%d" % (mode, v, depth))
+                    st = d.last_stats()
+                    rows.append({"leaf_mode": mode, "slider": pg.SLIDER_NAMES[v], "depth": depth, "s": dt,
+                                 "leaves_per_s": n / dt, "leaf_kernel_ms": st["leaf_kernel_ms"],
+                                 "leaf_items": st["leaf_items"]})
+                    log("leaf_mode %d %-14s d%d %.4f s %7.1f G leaves/s leaf kernel %.2f ms"
+                        % (mode, pg.SLIDER_NAMES[v], depth, dt, n / dt / 1e9, st["leaf_kernel_ms"]))
+    return {"workload": "start position perft(7) and perft(8), each timed after one warm run", "rows": rows,
             "exact": True}
+
+
+# ---------------------------------------------------------------- counters
+def _read_counters(outdir):
+    """rocprofv3 counter CSV -> {kernel: {counter: sum}, '#dispatches': {kernel: n}}"""
+    sums, disp = {}, {}
+    for path in glob.glob(os.path.join(outdir, "**", "*counter_collection.csv"), recursive=True):
+        with open(path, newline="") as f:
+            for row in csv.DictReader(f):
+                k = row.get("Kernel_Name", "")
+                c = row.get("Counter_Name", "")
+                v = float(row.get("Counter_Value", "0") or 0)
+                sums.setdefault(k, {})
+                sums[k][c] = sums[k].get(c, 0.0) + v
+                disp.setdefault(k, set()).add(row.get("Dispatch_Id", row.get("Correlation_Id", "")))
+    return sums, {k: len(v) for k, v in disp.items()}
+
+
+def _find(sums, needle):
+    hits = [k for k in sums if needle in k]
+    return hits[0] if hits else None
+
+
+def leg_counters(pg, dev=None):
+    """Hardware counters of this workload, collected by this run: two
+    rocprofv3 --pmc passes over a child process that runs one bench step
+    (Kogge-Stone) and the calib leg.  Instructions per counted board are
+    deterministic, so the parent's HIP-event timing turns them into a rate."""
+    exe = shutil.which("rocprofv3")
+    if exe is None:
+        return {"measured": False, "reason": "rocprofv3 not found"}
+    passes = [["SQ_INSTS_VALU", "SQ_WAVES", "GRBM_GUI_ACTIVE", "FETCH_SIZE"],
+              ["WRITE_SIZE", "SQ_INSTS_LDS", "SQ_LDS_BANK_CONFLICT", "SQ_INSTS_SALU"]]
+    child = [sys.executable, os.path.join(ROOT, "bench.py"), "--steps", "1", "--warmup", "0", "--no-cpu",
+             "--legs", "calib", "--variant", "0"]
+    work = tempfile.mkdtemp(prefix="pg_counters_", dir="/tmp")
+    sums, disp, child_line = {}, {}, None
+    env = dict(os.environ, TMPDIR="/tmp")
+    try:
+        for i, ctrs in enumerate(passes):
+            out = os.path.join(work, "pass%d" % i)
+            cmd = ["timeout", "-s", "KILL", "240", exe, "--pmc"] + ctrs + ["-d", out, "-o", "pmc", "--output-format",
+                                                                          "csv", "--"] + child
+            log("counters pass %d: %s" % (i + 1, " ".join(ctrs)))
+            r = subprocess.run(cmd, capture_output=True, text=True, cwd="/tmp", env=env)
+            lines = [x for x in r.stdout.splitlines() if x.startswith("{")]
+            if lines:
+                child_line = json.loads(lines[-1])
+            s_, d_ = _read_counters(out)
+            if not s_:
+                return {"measured": False, "reason": "rocprofv3 pass %d gave no counters (rc %d): %s"
+                        % (i + 1, r.returncode, (r.stderr or "")[-300:])}
+            for k, v in s_.items():
+                sums.setdefault(k, {}).update(v)
+            for k, v in d_.items():
+                disp[k] = max(disp.get(k, 0), v)
+    finally:
+        shutil.rmtree(work, ignore_errors=True)
+    leaf, copy = _find(sums, LEAF_KERNEL), _find(sums, "k_copy16")
+    if not leaf or not copy or child_line is None:
+        return {"measured": False, "reason": "leaf or copy kernel missing from the counter output"}
+    L, C = sums[leaf], sums[copy]
+    tot = child_line["leaf_totals"]
+    nl, nc = disp[leaf], disp[copy]
+    # FETCH_SIZE / WRITE_SIZE are KiB per dispatch; the copy reads and writes COPY_BYTES per dispatch
+    ff = COPY_BYTES * nc / (C["FETCH_SIZE"] * 1024.0)
+    wf = COPY_BYTES * nc / (C["WRITE_SIZE"] * 1024.0)
+    traffic = (L["FETCH_SIZE"] * ff + L["WRITE_SIZE"] * wf) * 1024.0 / nl
+    cus = child_line["calib"]["cus"]
+    cycles = L["GRBM_GUI_ACTIVE"] / 8.0    # GRBM_GUI_ACTIVE sums the eight XCDs
+    res = {"measured": True, "kernel": leaf, "dispatches": nl,
+           "valu_wave_insts": L["SQ_INSTS_VALU"], "boards": tot["items"],
+           "valu_insts_per_board": L["SQ_INSTS_VALU"] / tot["items"],
+           "valu_wave_insts_per_cu_per_cycle": L["SQ_INSTS_VALU"] / (cus * cycles),
+           "lds_insts": L.get("SQ_INSTS_LDS"), "lds_bank_conflict_cycles": L.get("SQ_LDS_BANK_CONFLICT"),
+           "salu_insts": L.get("SQ_INSTS_SALU"), "waves": L.get("SQ_WAVES"),
+           "hbm_bytes_per_launch": traffic, "algorithmic_bytes_per_launch": 52.0 * tot["parents"] / tot["launches"],
+           "copy_calibration": {"bytes_per_dispatch": COPY_BYTES, "dispatches": nc,
+                                "fetch_size_kib_per_dispatch": C["FETCH_SIZE"] / nc,
+                                "write_size_kib_per_dispatch": C["WRITE_SIZE"] / nc,
+                                "fetch_factor": ff, "write_factor": wf}}
+    probes = {}
+    for name in ("k_issue<0>", "k_issue<1>", "k_issue<2>", "k_issue<3>"):
+        k = _find(sums, name)
+        if k and sums[k].get("GRBM_GUI_ACTIVE"):
+            probes[name] = {"valu_wave_insts": sums[k]["SQ_INSTS_VALU"], "dispatches": disp[k],
+                            "wave_insts_per_cu_per_cycle": sums[k]["SQ_INSTS_VALU"]
+                            / (cus * sums[k]["GRBM_GUI_ACTIVE"] / 8.0)}
+    res["issue_probes"] = probes
+    log("counters: %.0f VALU instructions per board, %.3f per CU per cycle, %.1f B HBM per parent"
+        % (res["valu_insts_per_board"], res["valu_wave_insts_per_cu_per_cycle"], traffic * nl / tot["parents"]))
+    return res
 
 
 def leg_serve(pg, dev):
@@ -226,16 +377,24 @@ def main():
     ap.add_argument("--variant", type=int, default=-2, help="slider variant; -2 = pick the fastest")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-threads", type=int, default=min(16, os.cpu_count() or 1))
-    ap.add_argument("--legs", default="batch,tt,serve", help="extra legs beside the headline (comma list or none)")
+    ap.add_argument("--legs", default="counters,calib,batch,tt,serve",
+                    help="legs beside the headline (comma list, or none): counters, calib, batch, tt, serve, leafmodes")
+    ap.add_argument("--tt-gb", default="16,64", help="table sizes of the tt leg (GB, comma list)")
+    ap.add_argument("--tt-depths", default="9,10", help="depths of the tt leg (comma list)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU: this build runs perft on a single MI355X "
               "(--gpus %d requested); multi-GPU sharding is a design note only." % args.gpus)
         sys.exit(2)
+    legs_wanted = [x for x in args.legs.split(",") if x and x != "none"]
 
     import perft_gpu_amd as pg
+    # the counter passes run in child processes before this one touches the GPU
+    counters = leg_counters(pg) if "counters" in legs_wanted else None
+
     jobs = load_workload(pg)
     dev = pg.Device(0, arena_bytes=48 << 30, timing=True)
+    _, _, _, clk_khz, cus = dev.issue_rate(0, 16)
 
     # per-variant pass (the slider choice for this path); skipped when a
     # variant is forced (profiling runs)
@@ -259,47 +418,43 @@ def main():
     elapsed = max(time.perf_counter() - t0, 1e-9)
     value = leaves / elapsed
 
-    avg_ms = acc["ms"] / max(1, acc["launches"])
+    launches = max(1, acc["launches"])
+    avg_ms = acc["ms"] / launches
     fused = best != pg.SLIDER_MAGIC_BYTE   # the byte-lookup variant keeps the materialized leaf level
     if fused:
         # items live in LDS: a 48-byte board and a 4-byte move count per parent
-        bytes_per_launch = 52 * acc["parents"] / max(1, acc["launches"])
+        bytes_per_launch = 52 * acc["parents"] / launches
         kname, bnote = "k_leaf_fused", "52 B per parent board (board + move count); the (parent, move) items stay in LDS"
     else:
-        bytes_per_launch = (6 * acc["items"] + 48 * acc["parents"]) / max(1, acc["launches"])
+        bytes_per_launch = (6 * acc["items"] + 48 * acc["parents"]) / launches
         kname, bnote = "k_leaf", "6 B per (parent, move) item + 48 B per parent board"
-    achieved = bytes_per_launch / (avg_ms * 1e-3) / 1e9 if avg_ms > 0 else 0.0
-    traffic = None
-    if fused and best == pg.SLIDER_KOGGE_STONE:
-        traffic = TRAFFIC_BYTES_PER_PARENT * acc["parents"] / max(1, acc["launches"])
-    roofline = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
+    hbm_gbs = bytes_per_launch / (avg_ms * 1e-3) / 1e9 if avg_ms > 0 else 0.0
+    boards_per_s = acc["items"] / (acc["ms"] * 1e-3) if acc["ms"] > 0 else 0.0
+    # VALU roofline: every counted board costs a fixed number of VALU wave
+    # instructions (64 lanes each); the vector peak is CUs x 128 lanes x clock,
+    # the published 157.3 TFLOP/s fp32 vector rate counted in lane operations
+    # (an FMA is two flops).  Instructions per board come from the counters leg
+    # of this run; without rocprofv3 the committed profile's figure is used.
+    peak_tops = cus * 128 * clk_khz * 1e3 / 1e12
+    have_ctr = bool(counters and counters.get("measured")) and best == pg.SLIDER_KOGGE_STONE
+    ipb = counters["valu_insts_per_board"] if have_ctr else (
+        PROFILE_VALU_INSTS_PER_BOARD if best == pg.SLIDER_KOGGE_STONE else None)
+    achieved = boards_per_s * ipb * 64 / 1e12 if ipb else None
+    # (the child ran one step of the same workload: the same launches)
+    traffic = counters["hbm_bytes_per_launch"] if have_ctr else None
+    roofline = {"bound": "valu", "achieved": achieved, "peak": peak_tops, "unit": "T int32 lane-ops/s",
+                "frac": achieved / peak_tops if achieved else None, "traffic": traffic,
                 "kernel": "%s<%s>" % (kname, pg.SLIDER_NAMES[best]), "launches": acc["launches"],
-                "avg_launch_ms": avg_ms, "items_per_launch": acc["items"] / max(1, acc["launches"]),
-                "bytes_per_launch": bytes_per_launch,
-                "note": "integer VALU-bound kernel; algorithmic bytes: " + bnote}
-    items_per_s = acc["items"] / (acc["ms"] * 1e-3) if acc["ms"] > 0 else 0.0
-    # VALU issue: gfx950 issues the full-rate (VOP3) class -- bitop3, 64-bit
-    # shifts, popcount -- at one wave instruction per CU per clock and simple
-    # two-operand logic/move ops at up to two (a micro-benchmark measured 0.95
-    # and 1.6-1.7); the leaf's mix is mostly the former
-    valu = {"bound": "valu", "unit": "wave instructions per CU per clock (256 CUs, 2.4 GHz)", "peak": 2.0,
-            "single_issue_rate": 1.0, "items_per_s": items_per_s,
-            "note": "the kernel's real bound; counted boards per second from the HIP-event time above, "
-                    "instructions per board from the rocprofv3 counter profile"}
-    if best == pg.SLIDER_KOGGE_STONE:
-        valu["lane_insts_per_item"] = VALU_LANE_INSTS_PER_ITEM
-        valu["achieved"] = items_per_s * VALU_LANE_INSTS_PER_ITEM / 64 / (256 * 2.4e9)
-        valu["frac"] = valu["achieved"] / valu["peak"]
-
-    legs = {}
-    for leg in [x for x in args.legs.split(",") if x and x != "none"]:
-        legs[leg] = {"batch": leg_batch, "tt": leg_tt, "serve": leg_serve}[leg](pg, dev)
-
-    cpu = None
-    if not args.no_cpu:
-        cpu = cpu_baseline(jobs, args.cpu_threads)
-        log("cpu baseline: %.2f G leaves/s on %d threads" % (cpu["value"] / 1e9, cpu["cores"]))
+                "avg_launch_ms": avg_ms, "boards_per_launch": acc["items"] / launches,
+                "valu_wave_insts_per_board": ipb,
+                "valu_source": ("rocprofv3 --pmc SQ_INSTS_VALU in this run (counters leg)" if have_ctr
+                                else "%s (counters leg not run)" % PROFILE_SOURCE),
+                "wave_insts_per_cu_per_clock": achieved * 1e12 / 64 / (cus * clk_khz * 1e3) if achieved else None,
+                "peak_note": "%d CUs x 128 lanes x %.0f MHz (= published 157.3 TFLOP/s fp32 vector / 2); the issue "
+                             "probes (calib leg) show what each instruction class sustains" % (cus, clk_khz / 1e3),
+                "hbm": {"achieved": hbm_gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": hbm_gbs / HBM_PEAK_GBS,
+                        "algorithmic_bytes_per_launch": bytes_per_launch, "note": bnote,
+                        "measured_bytes_per_launch": traffic}}
 
     line = {
         "metric": METRIC, "value": value, "unit": "leaves/s", "n_gpus": 1, "steps": args.steps,
@@ -308,14 +463,32 @@ def main():
         "config": {"workload": "plain GPU BFS perft, no TT: start position d7 + d8 and 16 seeded random "
                                "positions (ply 10-60) at d6", "leaves_per_step": leaves // max(1, args.steps),
                    "slider": pg.SLIDER_NAMES[best], "exact": True},
-        "variants": variants, "roofline": roofline, "valu": valu, "cpu_baseline": cpu,
+        "variants": variants, "roofline": roofline,
+        "leaf_totals": {"launches": acc["launches"], "items": acc["items"], "parents": acc["parents"],
+                        "ms": acc["ms"]},
     }
-    line.update(legs)
-    if "serve" in legs:
-        r1k = [r for r in legs["serve"]["rates"] if r["rate_per_s"] == 1000][0]
+    if counters is not None:
+        line["counters"] = counters
+    for leg in legs_wanted:
+        if leg == "counters":
+            continue
+        if leg == "tt":
+            line["tt"] = leg_tt(pg, dev, tuple(int(x) for x in args.tt_gb.split(",")),
+                                tuple(int(x) for x in args.tt_depths.split(",")))
+        else:
+            line[leg] = {"batch": leg_batch, "serve": leg_serve, "calib": leg_calib,
+                         "leafmodes": leg_leafmodes}[leg](pg, dev)
+    dev.close()
+
+    cpu = None
+    if not args.no_cpu:
+        cpu = cpu_baseline(jobs, args.cpu_threads)
+        log("cpu baseline: %.2f G leaves/s on %d threads" % (cpu["value"] / 1e9, cpu["cores"]))
+    line["cpu_baseline"] = cpu
+    if "serve" in line:
+        r1k = [r for r in line["serve"]["rates"] if r["rate_per_s"] == 1000][0]
         line["served_p50_ms"] = r1k["p50_ms"]
         line["served_p99_ms"] = r1k["p99_ms"]
-    dev.close()
     print(json.dumps(line))
 
 

@@ -120,6 +120,12 @@ int pg_set_slider(pg_ctx *ctx, int slider);
  * breadth-first perft of one position; returns the leaf count. */
 int pg_perft(pg_ctx *ctx, const pg_pos48 *pos, uint32_t depth, int slider, uint64_t *out);
 int pg_last_stats(pg_ctx *ctx, pg_stats *out);
+/* Test hook (no reference counterpart; the reference sums deep totals in
+ * InfInt, launcher.h:1181): every later perft call of this context starts its
+ * root total at `bias` instead of 0, until set back to 0.  A total that then
+ * passes 2^64 must make the call return PG_EOVERFLOW, never a wrapped count;
+ * biased totals are never stored in a transposition table. */
+int pg_test_count_bias(pg_ctx *ctx, uint64_t bias);
 /* Text of the last error on the calling thread (empty if none). */
 const char *pg_last_error(void);
 
@@ -208,10 +214,23 @@ int pg_server_slots(pg_server *srv, uint32_t *slots, uint64_t *bytes_per_slot);
 /* hipMemGetInfo of the context's device (admission by free HBM). */
 int pg_device_memory(pg_ctx *ctx, uint64_t *free_bytes, uint64_t *total_bytes);
 
-/* Plain device-to-device copy of `bytes` (16-byte lanes), `iters` times;
- * *ms gets the mean launch time.  No reference counterpart: it calibrates
- * the HBM byte counters and the achievable copy bandwidth. */
+/* ---------------------------------------------------------- diagnostics */
+/* Measurement reference points (no reference counterpart; the reference's
+ * only timing is the wall clock around a perft, launcher.h:1486-1498).
+ *
+ * pg_device_copy: plain device-to-device copy of `bytes` (16-byte lanes),
+ * `iters` times; *ms gets the mean launch time.  Its FETCH_SIZE / WRITE_SIZE
+ * counters calibrate those of the perft kernels, its time the achievable
+ * copy bandwidth.
+ *
+ * pg_device_issue_probe: one launch of an unrolled stream of one VALU
+ * instruction class -- kind 0 v_bitop3_b32, 1 v_xor_b32 (two-operand
+ * encoding), 2 v_lshlrev_b64, 3 v_bcnt_u32_b32 -- with eight waves per SIMD;
+ * returns its time, the wave instructions it issued, the engine clock (kHz)
+ * and the CU count, from which the issue rate per CU per clock follows. */
 int pg_device_copy(pg_ctx *ctx, uint64_t bytes, uint32_t iters, double *ms);
+int pg_device_issue_probe(pg_ctx *ctx, int kind, uint32_t iters, double *ms, uint64_t *wave_insts,
+                          uint32_t *clock_khz, uint32_t *cus);
 
 #ifdef __cplusplus
 }

@@ -114,6 +114,7 @@ def lib():
         "pg_set_slider": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int]),
         "pg_perft": (ctypes.c_int, [ctypes.c_void_p, P(Pos48), ctypes.c_uint32, ctypes.c_int, P(u64)]),
         "pg_last_stats": (ctypes.c_int, [ctypes.c_void_p, P(Stats)]),
+        "pg_test_count_bias": (ctypes.c_int, [ctypes.c_void_p, u64]),
         "pg_last_error": (ctypes.c_char_p, []),
         "pg_device_count_moves": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_uint32, ctypes.c_int,
                                                  ctypes.c_void_p]),
@@ -124,6 +125,8 @@ def lib():
         "pg_device_slider_attacks": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.c_void_p,
                                                     ctypes.c_void_p, ctypes.c_uint32, ctypes.c_void_p]),
         "pg_device_copy": (ctypes.c_int, [ctypes.c_void_p, u64, ctypes.c_uint32, P(ctypes.c_double)]),
+        "pg_device_issue_probe": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.c_uint32, P(ctypes.c_double),
+                                                 P(u64), P(ctypes.c_uint32), P(ctypes.c_uint32)]),
         "pg_tt_create": (ctypes.c_int, [ctypes.c_void_p, u64, P(ctypes.c_void_p)]),
         "pg_tt_clear": (ctypes.c_int, [ctypes.c_void_p]),
         "pg_tt_destroy": (None, [ctypes.c_void_p]),
@@ -278,6 +281,10 @@ class Device:
     def set_dedup(self, on):
         _check(lib().pg_set_dedup(self._h, 1 if on else 0), "pg_set_dedup")
 
+    def set_count_bias(self, bias):
+        """Test hook pg_test_count_bias: later perft totals start at `bias`."""
+        _check(lib().pg_test_count_bias(self._h, bias), "pg_test_count_bias")
+
     def set_slider(self, slider):
         _check(lib().pg_set_slider(self._h, slider), "pg_set_slider")
 
@@ -339,6 +346,16 @@ class Device:
         ms = ctypes.c_double()
         _check(lib().pg_device_copy(self._h, nbytes, iters, ctypes.byref(ms)), "pg_device_copy")
         return ms.value, 2 * nbytes / (ms.value * 1e-3) / 1e9
+
+    ISSUE_KINDS = {0: "v_bitop3_b32", 1: "v_xor_b32 (two-operand)", 2: "v_lshlrev_b64", 3: "v_bcnt_u32_b32"}
+
+    def issue_rate(self, kind, iters=4096):
+        """VALU issue probe: (wave instructions per CU per clock, ms, instructions, clock kHz, CUs)."""
+        ms, n, clk, cus = ctypes.c_double(), u64(), ctypes.c_uint32(), ctypes.c_uint32()
+        _check(lib().pg_device_issue_probe(self._h, kind, iters, ctypes.byref(ms), ctypes.byref(n), ctypes.byref(clk),
+                                           ctypes.byref(cus)), "pg_device_issue_probe")
+        rate = n.value / (cus.value * ms.value * 1e-3 * clk.value * 1e3)
+        return rate, ms.value, int(n.value), int(clk.value), int(cus.value)
 
 
 class TranspositionTable:

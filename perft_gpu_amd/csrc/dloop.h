@@ -36,9 +36,11 @@ struct DLevel {
 struct Plan {
     u8 *base;
     u64 cap, top;
-    u32 overflow, err;
+    u32 overflow, err;  // overflow: a level did not fit the region
     u64 leafItems;
     u64 acc;
+    u64 bias;           // test hook: starting value of every root's total
+    u32 ovf;            // a 64-bit total wrapped
     DLevel lv[TT_MAX_DEPTH];
 };
 
@@ -83,7 +85,7 @@ static __global__ __launch_bounds__(256) void kd_root_probe(Plan *P, TTView tt, 
 
 static __global__ __launch_bounds__(256) void kd_root_store(Plan *P, TTView tt, int D, TTCounters *ctr) {
     const DLevel L = P->lv[0];
-    if (P->overflow) return;
+    if (P->overflow || P->bias) return;  // a biased test total is not a perft
     PG_GRID_LOOP(i, L.n) {
         const bool st = D >= 2 && L.k && L.state[i] == ST_LIVE && tt_store(tt, D, L.k[i], L.perft[i]);
         if (ctr) wave_count(&ctr->c[D][2], st);
@@ -344,15 +346,26 @@ __global__ __launch_bounds__(LEAF_WG) __attribute__((amdgpu_waves_per_eu(V == SL
             __syncthreads();
             start = wend;
         }
-        if (live && L.state[i] == ST_LIVE) L.perft[i] = sper[tid];
+        if (live && L.state[i] == ST_LIVE) {
+            u64 v = sper[tid];
+            if (l == 0 && P->bias) {
+                v += P->bias;
+                if (v < P->bias) P->ovf = 1;
+            }
+            L.perft[i] = v;
+        }
         __syncthreads();
     }
 }
 
 static __global__ __launch_bounds__(256) void kd_perft1(Plan *P, int l) {
     const DLevel L = P->lv[l];
+    const u64 base = l == 0 ? P->bias : 0;
     PG_GRID_LOOP(i, L.n) {
-        if (L.state[i] == ST_LIVE) L.perft[i] = L.cnt[i];
+        if (L.state[i] != ST_LIVE) continue;
+        const u64 v = base + L.cnt[i];
+        if (v < base) P->ovf = 1;
+        L.perft[i] = v;
     }
 }
 
@@ -374,11 +387,10 @@ static __global__ __launch_bounds__(256) void kd_segsum(Plan *P, int l) {
     const DLevel L = P->lv[l];
     const DLevel C = P->lv[l + 1];
     if (P->overflow) return;
+    const u64 base = l == 0 ? P->bias : 0;
     PG_GRID_LOOP(i, L.n) {
         if (L.state[i] != ST_LIVE) continue;
-        u64 s = 0;
-        for (u64 j = L.off[i]; j < L.off[i + 1]; j++) s += C.perft[j];
-        L.perft[i] = s;
+        L.perft[i] = sum_checked(C.perft, L.off[i], L.off[i + 1], base, &P->ovf);
     }
 }
 

@@ -47,6 +47,8 @@ struct Engine {
     u64 *dItems = nullptr;   // items consumed by fused leaf launches
     int leafMode = 0;        // 0 fused last two plies, 1 materialized item list
     u32 *dErr = nullptr;
+    u32 *dOvf = nullptr;     // set when a 64-bit total wraps
+    u64 bias = 0;            // test hook: starting value of every root total
     u64 *hostBuf = nullptr;  // pinned
     pg_stats stats{};
 
@@ -60,6 +62,7 @@ struct Engine {
     int bfs(int v, const Pos *boards, u64 n, const u32 *cnt, int r, int ply);
     int split(int v, const Pos *boards, u64 n, const u32 *cnt, int r, int ply);
     int perft(const Pos &root, u32 depth, int v, u64 *out);
+    int check_flags(const char *what);  // consistency and overflow flags of the last call
 
     // batched searches (graph.hip)
     Slot *batchSlot = nullptr;

@@ -74,7 +74,7 @@ template <int V> static void launch_count(Engine &c, const Pos *b, u64 n, u32 *c
 }
 template <int V> static void launch_count_sum(Engine &c, const Pos *b, u64 n, u64 *acc) {
     constexpr int W = wg_size<V>();
-    k_count_sum<V><<<c.grid(n, W), W, 0, c.stream>>>(b, n, acc, c.dt);
+    k_count_sum<V><<<c.grid(n, W), W, 0, c.stream>>>(b, n, acc, c.dOvf, c.dt);
 }
 template <int V> static void launch_generate(Engine &c, const Pos *b, u64 n, const u64 *off, u16 *mv, u32 *par) {
     constexpr int W = wg_size<V>();
@@ -86,7 +86,7 @@ template <int V> static void launch_expand(Engine &c, const Pos *b, const u32 *p
 }
 template <int V> static void launch_leaf(Engine &c, const Pos *b, const u32 *par, const u16 *mv, u64 m, u64 *acc) {
     constexpr int W = wg_size<V>();
-    k_leaf<V><<<c.grid(m, W), W, 0, c.stream>>>(b, par, mv, m, acc, c.dt);
+    k_leaf<V><<<c.grid(m, W), W, 0, c.stream>>>(b, par, mv, m, acc, c.dOvf, c.dt);
 }
 
 int Engine::scan(const u32 *cnt, u64 n, u64 *off, u64 *total) {
@@ -124,7 +124,8 @@ template <int V> static void launch_leaf_fused(Engine &c, const Pos *b, const u3
     if constexpr (V == SL_BYTE) {
         (void)c; (void)b; (void)cnt; (void)n;
     } else {
-        k_leaf_fused<V, false><<<c.leaf_grid(n), LEAF_WG, 0, c.stream>>>(b, cnt, n, c.dAcc, c.dItems, nullptr, c.dErr, c.dt);
+        k_leaf_fused<V, false><<<c.leaf_grid(n), LEAF_WG, 0, c.stream>>>(b, cnt, n, c.dAcc, c.dItems, nullptr, c.dErr,
+                                                                         c.dOvf, c.dt);
     }
 }
 
@@ -149,7 +150,7 @@ int Engine::leaf_fused(int v, const Pos *b, const u32 *cnt, u64 n) {
 int Engine::bfs(int v, const Pos *boards, u64 n, const u32 *cnt, int r, int ply) {
     if (n == 0) return PG_OK;
     if (r == 1) {
-        if (cnt) k_sum_u32<<<grid(n, 256), 256, 0, stream>>>(cnt, n, dAcc);
+        if (cnt) k_sum_u32<<<grid(n, 256), 256, 0, stream>>>(cnt, n, dAcc, dOvf);
         else PG_VARIANT_CALL(v, launch_count_sum, *this, boards, n, dAcc);
         HIP_TRY(hipGetLastError());
         return PG_OK;
@@ -204,6 +205,23 @@ int Engine::bfs(int v, const Pos *boards, u64 n, const u32 *cnt, int r, int ply)
     return rc;
 }
 
+// PG_EOVERFLOW if a 64-bit total wrapped, PG_EINVAL if a device
+// consistency check failed (the stream has been synchronized by the caller's
+// read of the result).
+int Engine::check_flags(const char *what) {
+    u32 f[2] = {0, 0};
+    HIP_TRY(hipMemcpy(f, dErr, sizeof(f), hipMemcpyDeviceToHost));
+    if (f[1]) {
+        set_error("%s: a perft total does not fit 64 bits", what);
+        return PG_EOVERFLOW;
+    }
+    if (f[0]) {
+        set_error("%s: device consistency check failed (flag %u)", what, f[0]);
+        return PG_EINVAL;
+    }
+    return PG_OK;
+}
+
 int Engine::split(int v, const Pos *boards, u64 n, const u32 *cnt, int r, int ply) {
     if (n <= 1) {
         set_error("arena too small for a single board at ply %d", ply);
@@ -225,19 +243,14 @@ int Engine::perft(const Pos &root, u32 depth, int v, u64 *out) {
     Pos *d = (Pos *)alloc(sizeof(Pos));
     if (!d) return PG_ENOMEM;
     HIP_TRY(hipMemcpyAsync(d, &root, sizeof(Pos), hipMemcpyHostToDevice, stream));
-    HIP_TRY(hipMemsetAsync(dAcc, 0, 2 * sizeof(u64), stream));
-    HIP_TRY(hipMemsetAsync(dErr, 0, sizeof(u32), stream));
+    k_init_call<<<1, 64, 0, stream>>>(dAcc, bias, dItems, dErr, dOvf);
     int rc = bfs(v, d, 1, nullptr, (int)depth, 0);
     if (rc) return rc;
     u64 res = 0;
     rc = read_u64(dAcc, &res);
     if (rc) return rc;
-    u32 err = 0;
-    HIP_TRY(hipMemcpy(&err, dErr, sizeof(u32), hipMemcpyDeviceToHost));
-    if (err) {
-        set_error("move generator count/generate mismatch (flag %u)", err);
-        return PG_EINVAL;
-    }
+    rc = check_flags("perft");
+    if (rc) return rc;
     if (leafMode == 0) {
         u64 items = 0;
         rc = read_u64(dItems, &items);
@@ -316,6 +329,7 @@ int pg_init(int device, const pg_config *cfg, pg_ctx **out) {
     c->dItems = (u64 *)(c->dSmall + 8);
     c->leafMode = c->cfg.leaf_mode;
     c->dErr = (u32 *)(c->dSmall + 64);
+    c->dOvf = (u32 *)(c->dSmall + 68);
     if (hipHostMalloc(&c->hostBuf, 64) != hipSuccess) return fail(PG_ENOMEM);
     size_t freeB = 0, totalB = 0;
     (void)hipMemGetInfo(&freeB, &totalB);
@@ -374,6 +388,12 @@ int pg_perft(pg_ctx *c, const pg_pos48 *pos, uint32_t depth, int slider, uint64_
         return PG_EINVAL;
     }
     return c->perft(p, depth, v, reinterpret_cast<u64 *>(out));
+}
+
+int pg_test_count_bias(pg_ctx *c, uint64_t bias) {
+    if (!c) return PG_EINVAL;
+    c->bias = bias;
+    return PG_OK;
 }
 
 int pg_last_stats(pg_ctx *c, pg_stats *out) {
@@ -483,26 +503,6 @@ int pg_device_slider_attacks(pg_ctx *c, int slider, int rook, const uint8_t *sq,
     PG_VARIANT_CALL(v, launch_probe_slider, *c, rook, (const u8 *)ds, (const u64 *)dO, n, (u64 *)dout);
     if (hipGetLastError() != hipSuccess) return PG_EHIP;
     PROBE_TRY(stage_out(c, out, dout, n * sizeof(u64)));
-    c->top = 0;
-    return PG_OK;
-}
-
-int pg_device_copy(pg_ctx *c, uint64_t bytes, uint32_t iters, double *ms_out) {
-    if (!c || bytes < 16 || iters == 0) return PG_EINVAL;
-    (void)hipSetDevice(c->device);
-    c->top = 0;
-    const u64 n = bytes / 16;
-    void *a = c->alloc(n * 16), *b = c->alloc(n * 16);
-    if (!a || !b) { c->top = 0; return PG_ENOMEM; }
-    if (hipMemsetAsync(a, 1, n * 16, c->stream) != hipSuccess) return PG_EHIP;
-    const u32 g = c->grid(n, 256);
-    if (hipEventRecord(c->ev0, c->stream) != hipSuccess) return PG_EHIP;
-    for (uint32_t i = 0; i < iters; i++) k_copy16<<<g, 256, 0, c->stream>>>((const uint4 *)a, (uint4 *)b, n);
-    if (hipEventRecord(c->ev1, c->stream) != hipSuccess) return PG_EHIP;
-    if (hipEventSynchronize(c->ev1) != hipSuccess) return PG_EHIP;
-    float ms = 0;
-    (void)hipEventElapsedTime(&ms, c->ev0, c->ev1);
-    if (ms_out) *ms_out = ms / iters;
     c->top = 0;
     return PG_OK;
 }

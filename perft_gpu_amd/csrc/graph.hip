@@ -129,6 +129,7 @@ static int run_range(Engine &e, Slot &sl, const GraphShape &sh, u64 first, u64 n
     h.base = sl.region;
     h.cap = sl.regionBytes;
     h.top = top0;
+    h.bias = e.bias;
     h.lv[0] = L0;
     h.lv[0].n = n;
     h.lv[0].b += first;
@@ -151,6 +152,10 @@ static int run_range(Engine &e, Slot &sl, const GraphShape &sh, u64 first, u64 n
     *ms += t;
     *overflow = h.overflow != 0;
     *leafItems += h.leafItems;
+    if (h.ovf) {
+        set_error("batch: a perft total does not fit 64 bits");
+        return PG_EOVERFLOW;
+    }
     if (h.err) {
         set_error("device consistency check failed in batch (flag %u)", h.err);
         return PG_EINVAL;
@@ -467,6 +472,7 @@ int pg_server_submit(pg_server *s, uint32_t slot, const pg_pos48 *pos, uint32_t 
     h.base = sl.region;
     h.cap = sl.regionBytes;
     h.top = top;
+    h.bias = e.bias;
     h.lv[0] = L0;
     HIP_TRY(hipMemcpyAsync(sl.dPlan, &h, sizeof(Plan), hipMemcpyHostToDevice, sl.stream));
     HIP_TRY(hipGraphLaunch(exec, sl.stream));
@@ -498,6 +504,10 @@ int pg_server_poll(pg_server *s, uint32_t slot, uint64_t *out, pg_batch_stats *s
     Engine &e = *s->eng;
     Slot &sl = ss.slot;
     const Plan &h = *sl.hPlan;
+    if (h.ovf) {
+        set_error("served batch: a perft total does not fit 64 bits");
+        return PG_EOVERFLOW;
+    }
     if (h.err) {
         set_error("device consistency check failed in served batch (flag %u)", h.err);
         return PG_EINVAL;

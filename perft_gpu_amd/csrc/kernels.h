@@ -26,6 +26,48 @@ __device__ __forceinline__ u64 wave_sum(u64 v) {
     return v;
 }
 
+// *acc += s, raising *ovf when the 64-bit total wraps (a perft count that
+// does not fit u64 is reported, never returned wrapped).  A wave's own sum
+// cannot wrap: it counts at most a few million moves per lane.
+__device__ __forceinline__ void add_checked(u64 *acc, u64 s, u32 *ovf) {
+    const u64 old = atomicAdd(acc, s);
+    if (old + s < old) *ovf = 1;
+}
+
+// sum of v[lo..hi) with a carry check
+__device__ __forceinline__ u64 sum_checked(const u64 *__restrict__ v, u64 lo, u64 hi, u64 base, u32 *ovf) {
+    u64 s = base;
+    bool carry = false;
+    for (u64 j = lo; j < hi; j++) {
+        const u64 x = v[j];
+        s += x;
+        carry |= s < x;
+    }
+    if (carry) *ovf = 1;
+    return s;
+}
+
+// per-call device state reset without memset nodes: total (with an optional
+// starting bias), consumed leaf items, error and overflow flags
+static __global__ void k_init_call(u64 *acc, u64 bias, u64 *items, u32 *err, u32 *ovf) {
+    if (threadIdx.x || blockIdx.x) return;
+    acc[0] = bias;
+    acc[1] = 0;
+    if (items) *items = 0;
+    *err = 0;
+    *ovf = 0;
+}
+
+// P[i] += base for n values, raising *ovf on a wrap (the count-bias test
+// hook where a root's perft is not produced by a checked sum)
+static __global__ void k_add_base(u64 *P, u64 n, u64 base, u32 *ovf) {
+    for (u64 i = threadIdx.x; i < n; i += blockDim.x) {
+        const u64 v = P[i] + base;
+        if (v < base) *ovf = 1;
+        P[i] = v;
+    }
+}
+
 template <class T>
 __device__ __forceinline__ void lds_copy(T *dst, const T *src, int count) {
     // 16-byte chunks; every table here is a multiple of 16 bytes
@@ -96,19 +138,19 @@ __global__ __launch_bounds__(wg_size<V>()) void k_count(const Pos *__restrict__ 
 }
 
 template <int V>
-__global__ __launch_bounds__(wg_size<V>()) void k_count_sum(const Pos *__restrict__ b, u64 n, u64 *acc, Tables t) {
+__global__ __launch_bounds__(wg_size<V>()) void k_count_sum(const Pos *__restrict__ b, u64 n, u64 *acc, u32 *ovf, Tables t) {
     stage_tables<V>(t);
     u64 s = 0;
     PG_GRID_LOOP(i, n) s += count_any<V>(b[i], t);
     s = wave_sum(s);
-    if ((threadIdx.x & 63) == 0 && s) atomicAdd(acc, s);
+    if ((threadIdx.x & 63) == 0 && s) add_checked(acc, s, ovf);
 }
 
-static __global__ __launch_bounds__(256) void k_sum_u32(const u32 *__restrict__ c, u64 n, u64 *acc) {
+static __global__ __launch_bounds__(256) void k_sum_u32(const u32 *__restrict__ c, u64 n, u64 *acc, u32 *ovf) {
     u64 s = 0;
     PG_GRID_LOOP(i, n) s += c[i];
     s = wave_sum(s);
-    if ((threadIdx.x & 63) == 0 && s) atomicAdd(acc, s);
+    if ((threadIdx.x & 63) == 0 && s) add_checked(acc, s, ovf);
 }
 
 // err: set to 1 if a board generates a different number of moves than it
@@ -143,12 +185,12 @@ __global__ __launch_bounds__(wg_size<V>()) void k_expand(const Pos *__restrict__
 
 template <int V>
 __global__ __launch_bounds__(wg_size<V>()) void k_leaf(const Pos *__restrict__ parents, const u32 *__restrict__ pidx,
-                                                       const u16 *__restrict__ moves, u64 m, u64 *acc, Tables t) {
+                                                       const u16 *__restrict__ moves, u64 m, u64 *acc, u32 *ovf, Tables t) {
     stage_tables<V>(t);
     u64 s = 0;
     PG_GRID_LOOP(i, m) s += count_after<V>(parents[pidx[i]], moves[i], t);
     s = wave_sum(s);
-    if ((threadIdx.x & 63) == 0 && s) atomicAdd(acc, s);
+    if ((threadIdx.x & 63) == 0 && s) add_checked(acc, s, ovf);
 }
 
 // ---------------------------------------------------------------- scan helpers
@@ -199,7 +241,7 @@ template <bool PER> constexpr int leaf_cap() { return PER ? 6848 : 7040; }
 
 template <int V, bool PER>
 __global__ __launch_bounds__(LEAF_WG) __attribute__((amdgpu_waves_per_eu(V == SL_KOGGE ? 4 : 5, 8))) void k_leaf_fused(const Pos *__restrict__ boards, const u32 *__restrict__ cnt, u64 n,
-                                                        u64 *acc, u64 *items, u64 *__restrict__ per, u32 *err, Tables t) {
+                                                        u64 *acc, u64 *items, u64 *__restrict__ per, u32 *err, u32 *ovf, Tables t) {
     static_assert(V != SL_BYTE, "the byte-lookup tables leave no LDS for the move window");
     __shared__ Pos sb[LEAF_WG];
     constexpr int LEAF_CAP = leaf_cap<PER>();
@@ -249,7 +291,7 @@ __global__ __launch_bounds__(LEAF_WG) __attribute__((amdgpu_waves_per_eu(V == SL
     }
     if (!PER) {
         s = wave_sum(s);
-        if ((tid & 63) == 0 && s) atomicAdd(acc, s);
+        if ((tid & 63) == 0 && s) add_checked(acc, s, ovf);
     }
 }
 
@@ -313,11 +355,6 @@ static __global__ __launch_bounds__(SCAN_WG) void k_scan_down(const u32 *__restr
 }
 
 // ---------------------------------------------------------------- probes
-// plain 16-byte copy of a known size: the reference point that scales the
-// HBM byte counters of the other kernels
-static __global__ __launch_bounds__(256) void k_copy16(const uint4 *__restrict__ src, uint4 *__restrict__ dst, u64 n) {
-    PG_GRID_LOOP(i, n) dst[i] = src[i];
-}
 
 template <int V>
 __global__ __launch_bounds__(wg_size<V>()) void k_probe_generate(const Pos *__restrict__ b, u64 n, u16 *__restrict__ moves,

@@ -195,15 +195,14 @@ static __global__ __launch_bounds__(256) void k_resolve_store(const Key128 *__re
     }
 }
 
-// P[i] = sum of the children's perfts of board i (children are contiguous);
-// boards resolved before expansion keep their value.
+// P[i] = base + sum of the children's perfts of board i (children are
+// contiguous; base is 0 except under the count-bias test hook); boards
+// resolved before expansion keep their value.  A sum that wraps raises *ovf.
 static __global__ __launch_bounds__(256) void k_segsum(const Pos *__restrict__ boards, u64 n, const u64 *__restrict__ off,
-                                                const u64 *__restrict__ cperft, u64 *__restrict__ P) {
+                                                const u64 *__restrict__ cperft, u64 *__restrict__ P, u64 base, u32 *ovf) {
     PG_GRID_LOOP(i, n) {
         if (!pos_valid(boards[i])) continue;
-        u64 s = 0;
-        for (u64 j = off[i]; j < off[i + 1]; j++) s += cperft[j];
-        P[i] = s;
+        P[i] = sum_checked(cperft, off[i], off[i + 1], base, ovf);
     }
 }
 
@@ -236,10 +235,14 @@ static __global__ __launch_bounds__(256) void k_fill_random(ulonglong2 *p, u64 n
     }
 }
 
-// perft(1) of valid boards = their move counts
-static __global__ __launch_bounds__(256) void k_perft1(const Pos *__restrict__ boards, u64 n, const u32 *__restrict__ cnt, u64 *P) {
+// perft(1) of valid boards = their move counts (+ base, see k_segsum)
+static __global__ __launch_bounds__(256) void k_perft1(const Pos *__restrict__ boards, u64 n, const u32 *__restrict__ cnt, u64 *P,
+                                                u64 base, u32 *ovf) {
     PG_GRID_LOOP(i, n) {
-        if (pos_valid(boards[i])) P[i] = cnt[i];
+        if (!pos_valid(boards[i])) continue;
+        const u64 v = base + cnt[i];
+        if (v < base) *ovf = 1;
+        P[i] = v;
     }
 }
 

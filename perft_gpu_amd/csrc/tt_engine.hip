@@ -27,7 +27,8 @@ template <int V> static void launch_expand_tt(Engine &c, TTable &tt, const Pos *
 }
 template <int V> static void launch_leaf_per(Engine &c, const Pos *B, const u32 *cnt, u64 n, u64 *P) {
     if constexpr (V != SL_BYTE)
-        k_leaf_fused<V, true><<<c.leaf_grid(n), LEAF_WG, 0, c.stream>>>(B, cnt, n, c.dAcc, c.dItems, P, c.dErr, c.dt);
+        k_leaf_fused<V, true><<<c.leaf_grid(n), LEAF_WG, 0, c.stream>>>(B, cnt, n, c.dAcc, c.dItems, P, c.dErr, c.dOvf,
+                                                                        c.dt);
 }
 template <int V> static void launch_gen(Engine &c, const Pos *b, u64 n, const u64 *off, u16 *mv, u32 *par) {
     constexpr int W = wg_size<V>();
@@ -60,13 +61,14 @@ int Engine::tt_split(int v, TTable &tt, Pos *B, Key128 *K, u64 n, u32 *cnt, int 
 int Engine::tt_bfs(int v, TTable &tt, Pos *B, Key128 *K, u64 n, u32 *cnt, int r, u64 *P, int ply) {
     if (n == 0) return PG_OK;
     if (r == 1) {
-        k_perft1<<<grid(n, 256), 256, 0, stream>>>(B, n, cnt, P);
+        k_perft1<<<grid(n, 256), 256, 0, stream>>>(B, n, cnt, P, ply == 0 ? bias : 0, dOvf);
         HIP_TRY(hipGetLastError());
         return PG_OK;
     }
     if (r == 2 && v != SL_BYTE) {
         if (timing) HIP_TRY(hipEventRecord(ev0, stream));
         PG_VARIANT_CALL(v, launch_leaf_per, *this, B, cnt, n, P);
+        if (ply == 0 && bias) k_add_base<<<1, 64, 0, stream>>>(P, n, bias, dOvf);
         HIP_TRY(hipGetLastError());
         if (timing) {
             HIP_TRY(hipEventRecord(ev1, stream));
@@ -126,7 +128,7 @@ int Engine::tt_bfs(int v, TTable &tt, Pos *B, Key128 *K, u64 n, u32 *cnt, int r,
         if (rc) return rc;
         k_resolve_store<<<grid(m, 256), 256, 0, stream>>>(CK, m, CP, S, D, tt.view, rcd, tt.dCtr);
     }
-    k_segsum<<<grid(n, 256), 256, 0, stream>>>(B, n, off, CP, P);
+    k_segsum<<<grid(n, 256), 256, 0, stream>>>(B, n, off, CP, P, ply == 0 ? bias : 0, dOvf);
     HIP_TRY(hipGetLastError());
     top = mark;
     return PG_OK;
@@ -151,12 +153,11 @@ int Engine::perft_tt(TTable &tt, const Pos &root, u32 depth, int v, u64 *out) {
     u64 *dP = (u64 *)(d + 128);
     HIP_TRY(hipMemcpyAsync(dB, &root, sizeof(Pos), hipMemcpyHostToDevice, stream));
     HIP_TRY(hipMemcpyAsync(dK, &key, sizeof(Key128), hipMemcpyHostToDevice, stream));
-    HIP_TRY(hipMemsetAsync(dP, 0, 8, stream));
-    HIP_TRY(hipMemsetAsync(dAcc, 0, 2 * sizeof(u64), stream));
-    HIP_TRY(hipMemsetAsync(dErr, 0, sizeof(u32), stream));
+    k_init_call<<<1, 64, 0, stream>>>(dP, 0, dItems, dErr, dOvf);
+    k_init_call<<<1, 64, 0, stream>>>(dAcc, 0, nullptr, dErr, dOvf);
     u64 res = 0;
     bool rootHit = false;
-    if (depth >= 2 && tt.view.base[depth]) {
+    if (depth >= 2 && tt.view.base[depth] && bias == 0) {
         k_tt_single<<<1, 64, 0, stream>>>(tt.view, (int)depth, key, dP, dHit, 0);
         u32 hit = 0;
         HIP_TRY(hipMemcpyAsync(hostBuf, dHit, 4, hipMemcpyDeviceToHost, stream));
@@ -165,21 +166,17 @@ int Engine::perft_tt(TTable &tt, const Pos &root, u32 depth, int v, u64 *out) {
         rootHit = hit != 0;
     }
     if (!rootHit) {
-        HIP_TRY(hipMemsetAsync(dP, 0, 8, stream));
         PG_VARIANT_CALL(v, launch_cnt, *this, dB, 1, dCnt);
         int rc = tt_bfs(v, tt, dB, dK, 1, dCnt, (int)depth, dP, 0);
         if (rc) return rc;
-        if (depth >= 2) k_tt_single<<<1, 64, 0, stream>>>(tt.view, (int)depth, key, dP, dHit, 1);
+        // (a biased test total is never stored: it is not the root's perft)
+        if (depth >= 2 && bias == 0) k_tt_single<<<1, 64, 0, stream>>>(tt.view, (int)depth, key, dP, dHit, 1);
         HIP_TRY(hipGetLastError());
     }
     int rc = read_u64(dP, &res);
     if (rc) return rc;
-    u32 err = 0;
-    HIP_TRY(hipMemcpy(&err, dErr, sizeof(u32), hipMemcpyDeviceToHost));
-    if (err) {
-        set_error("device consistency check failed (flag %u)", err);
-        return PG_EINVAL;
-    }
+    rc = check_flags("TT perft");
+    if (rc) return rc;
     u64 items = 0;
     rc = read_u64(dItems, &items);
     if (rc) return rc;

@@ -42,9 +42,10 @@ def test_c2_random_positions_depth6(dev, pg, v):
         assert dev.perft(pg.pos_from_words(item["words"]), 6, v) == item["perft"][6]
 
 
-def test_start_depth8(dev):
-    print("start d8", flush=True)
-    assert dev.perft(W.CPW["start"], 8, 0) == W.CPW_COUNTS["start"][8]
+@pytest.mark.parametrize("v", VARIANTS)
+def test_start_depth8(dev, v):
+    print("start d8 variant %d" % v, flush=True)
+    assert dev.perft(W.CPW["start"], 8, v) == W.CPW_COUNTS["start"][8]
     st = dev.last_stats()
     assert st["leaves"] == W.CPW_COUNTS["start"][8]
     assert st["nodes"][1:8] == W.CPW_COUNTS["start"][1:8]

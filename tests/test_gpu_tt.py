@@ -23,7 +23,7 @@ def test_tt_layout(tt):
     assert bits[2] == -1 and bits[3] == bits[4] > 0 and all(b > 0 for b in bits[5:])
 
 
-@pytest.mark.parametrize("v", [0, 2, 3])
+@pytest.mark.parametrize("v", [0, 1, 2, 3])
 def test_tt_start_position(dev, tt, v):
     tt.clear()
     for d in range(1, 9):
@@ -86,6 +86,17 @@ def test_tt_depth9_equals_no_tt_path(dev, pg):
     print("no-tt d9", flush=True)
     b = dev.perft(W.CPW["start"], 9, 0)
     assert a == b == W.CPW_COUNTS["start"][9]
+
+
+@pytest.mark.slow
+@pytest.mark.parametrize("gb", [16, 64])
+def test_tt_depth10_with_16_and_64_gb(pg, gb):
+    """TT perft(10) of the start position with 16 GB and 64 GB tables equals
+    the published 69,352,859,712,417 (perft(9) on the same table first)."""
+    with pg.Device(0, arena_bytes=48 << 30) as d, pg.TranspositionTable(d, gb << 30) as big:
+        for depth in (9, 10):
+            print("tt %d GB d%d" % (gb, depth), flush=True)
+            assert big.perft(W.CPW["start"], depth, 0) == W.CPW_COUNTS["start"][depth]
 
 
 def test_tt_checkpoint_resume(dev, pg, tmp_path):
