@@ -42,9 +42,10 @@ sys.path.insert(0, ROOT)
 
 METRIC = "perft leaves/sec on 1 MI355X (exact counts); p50/p99 latency of served queries"
 HBM_PEAK_GBS = 8000.0
-# Used only when the counters leg cannot run (no rocprofv3): VALU wave
-# instructions per counted board of k_leaf_fused<kogge_stone> measured by
-# rocprofv3 on this workload (profiles/k_leaf_fused_kogge_start_d7_d8_c2_d6.md)
+# Used only when the counters leg cannot run (no rocprofv3): VALU
+# instructions per counted board (SQ_INSTS_VALU x 64 / boards, one lane per
+# board) of k_leaf_fused<kogge_stone> measured by rocprofv3 on this workload
+# (profiles/k_leaf_fused_kogge_start_d7_d8_c2_d6.md)
 PROFILE_VALU_INSTS_PER_BOARD = 629
 PROFILE_SOURCE = "profiles/k_leaf_fused_kogge_start_d7_d8_c2_d6.md"
 LEAF_KERNEL = "k_leaf_fused<0, false>"
@@ -153,7 +154,7 @@ def leg_calib(pg, dev):
     ms, gbs = dev.copy_bandwidth(COPY_BYTES, COPY_ITERS)
     probes = {}
     for kind, name in dev.ISSUE_KINDS.items():
-        rate, pms, n, clk, cus = dev.issue_rate(kind, 4096)
+        rate, pms, n, clk, cus = dev.issue_rate(kind, 16384)
         probes[name] = {"wave_insts_per_cu_per_clock": rate, "ms": pms, "wave_insts": n}
         log("issue probe %-24s %.3f wave instructions per CU per clock" % (name, rate))
     log("copy %d MiB: %.3f ms, %.0f GB/s" % (COPY_BYTES >> 20, ms, gbs))
@@ -256,7 +257,9 @@ def leg_counters(pg, dev=None):
     cycles = L["GRBM_GUI_ACTIVE"] / 8.0    # GRBM_GUI_ACTIVE sums the eight XCDs
     res = {"measured": True, "kernel": leaf, "dispatches": nl,
            "valu_wave_insts": L["SQ_INSTS_VALU"], "boards": tot["items"],
-           "valu_insts_per_board": L["SQ_INSTS_VALU"] / tot["items"],
+           # each counted board is one lane: wave instructions x 64 / boards is
+           # the per-lane program length
+           "valu_insts_per_board": L["SQ_INSTS_VALU"] * 64.0 / tot["items"],
            "valu_wave_insts_per_cu_per_cycle": L["SQ_INSTS_VALU"] / (cus * cycles),
            "lds_insts": L.get("SQ_INSTS_LDS"), "lds_bank_conflict_cycles": L.get("SQ_LDS_BANK_CONFLICT"),
            "salu_insts": L.get("SQ_INSTS_SALU"), "waves": L.get("SQ_WAVES"),
@@ -439,14 +442,14 @@ def main():
     have_ctr = bool(counters and counters.get("measured")) and best == pg.SLIDER_KOGGE_STONE
     ipb = counters["valu_insts_per_board"] if have_ctr else (
         PROFILE_VALU_INSTS_PER_BOARD if best == pg.SLIDER_KOGGE_STONE else None)
-    achieved = boards_per_s * ipb * 64 / 1e12 if ipb else None
+    achieved = boards_per_s * ipb / 1e12 if ipb else None   # lane operations per second
     # (the child ran one step of the same workload: the same launches)
     traffic = counters["hbm_bytes_per_launch"] if have_ctr else None
     roofline = {"bound": "valu", "achieved": achieved, "peak": peak_tops, "unit": "T int32 lane-ops/s",
                 "frac": achieved / peak_tops if achieved else None, "traffic": traffic,
                 "kernel": "%s<%s>" % (kname, pg.SLIDER_NAMES[best]), "launches": acc["launches"],
                 "avg_launch_ms": avg_ms, "boards_per_launch": acc["items"] / launches,
-                "valu_wave_insts_per_board": ipb,
+                "valu_insts_per_board": ipb,
                 "valu_source": ("rocprofv3 --pmc SQ_INSTS_VALU in this run (counters leg)" if have_ctr
                                 else "%s (counters leg not run)" % PROFILE_SOURCE),
                 "wave_insts_per_cu_per_clock": achieved * 1e12 / 64 / (cus * clk_khz * 1e3) if achieved else None,

@@ -167,6 +167,35 @@ int pg_tt_counters(pg_tt *tt, uint64_t *out);
 /* perftLauncher, TT branch, driving perft_bb_gpu_simple_hash
  * (launcher.h:1457-1466, perft_bb.h:1242-1589) for one position. */
 int pg_perft_tt(pg_ctx *ctx, pg_tt *tt, const pg_pos48 *pos, uint32_t depth, int slider, uint64_t *out);
+/* perft_bb_cpu_launcher + dividedPerft (launcher.h:1181-1389, :1392-1450):
+ * perft of any depth, with a 128-bit total out[0] (low) / out[1] (high) --
+ * the reference keeps these in InfInt.  Nodes deeper than launch_depth are
+ * walked on the host (moves ordered quiet first, as sortMoves,
+ * launcher.h:367) and memoised in a host table keyed by (128-bit key,
+ * depth), the reference's complete TT (:598-697); each node at launch_depth
+ * is one device call (pg_perft_tt if tt is given, else pg_perft).  For every
+ * move of a host node deeper than divided_depth, cb (if not NULL) receives
+ * the move path ("..e2-e4 e7*d6 ", Utils::getCompactMoveString) and that
+ * child's total, in the order the reference prints them (launcher.h:1346). */
+typedef void (*pg_divide_cb)(void *user, const char *path, uint64_t lo, uint64_t hi);
+typedef struct pg_deep_stats {
+    double total_ms;        /* wall time of the call */
+    double device_ms;       /* time inside device perft calls (the reference's gpuTime) */
+    uint64_t device_calls;  /* device perft calls made */
+    uint64_t host_nodes;    /* host nodes visited */
+    uint64_t host_tt_hits;  /* host nodes answered by the host table */
+} pg_deep_stats;
+int pg_perft_deep(pg_ctx *ctx, pg_tt *tt, const pg_pos48 *pos, uint32_t depth, uint32_t launch_depth,
+                  uint32_t divided_depth, int slider, pg_divide_cb cb, void *user, uint64_t out[2],
+                  pg_deep_stats *stats);
+/* processPerftRecords (launcher.h:1578-1695): perft(depth) of every FEN line
+ * of in_path, whose last number is an occurrence count; appends
+ * "<line> <perft> <perft * count>" per line to <in_path>.op.  Lines the .op
+ * file already answers are skipped (resume, :1627-1631); the rest go to the
+ * device `chunk` lines per batched launch (pg_perft_batch, tt optional),
+ * each chunk flushed before the next. */
+int pg_perft_records(pg_ctx *ctx, pg_tt *tt, const char *in_path, uint32_t depth, int slider, uint32_t chunk,
+                     uint32_t *done, uint32_t *skipped);
 /* Test hook (no reference counterpart): overwrite every entry with seeded
  * pseudo-random words; lookups must then miss, never return garbage. */
 int pg_tt_poison(pg_tt *tt, uint64_t seed);

@@ -75,6 +75,17 @@ class Stats(ctypes.Structure):
         return d
 
 
+class DeepStats(ctypes.Structure):
+    _fields_ = [("total_ms", ctypes.c_double), ("device_ms", ctypes.c_double), ("device_calls", u64),
+                ("host_nodes", u64), ("host_tt_hits", u64)]
+
+    def as_dict(self):
+        return {n: getattr(self, n) for n, _ in self._fields_}
+
+
+DIVIDE_CB = ctypes.CFUNCTYPE(None, ctypes.c_void_p, ctypes.c_char_p, u64, u64)
+
+
 class PerftError(RuntimeError):
     def __init__(self, code, what):
         msg = "%s failed: %s" % (what, _ERR_NAMES.get(code, code))
@@ -138,6 +149,11 @@ def lib():
         "pg_perft_tt": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, P(Pos48), ctypes.c_uint32, ctypes.c_int,
                                        P(u64)]),
         "pg_set_dedup": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int]),
+        "pg_perft_deep": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, P(Pos48), ctypes.c_uint32, ctypes.c_uint32,
+                                         ctypes.c_uint32, ctypes.c_int, DIVIDE_CB, ctypes.c_void_p, P(u64),
+                                         P(DeepStats)]),
+        "pg_perft_records": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_char_p, ctypes.c_uint32,
+                                            ctypes.c_int, ctypes.c_uint32, P(ctypes.c_uint32), P(ctypes.c_uint32)]),
         "pg_perft_batch": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_uint32, ctypes.c_uint32,
                                           ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p,
                                           P(BatchStats)]),
@@ -424,6 +440,37 @@ def divide(dev, pos, depth, slider=SLIDER_AUTO, tt=None):
     kids = np.array([make_move_host(pos, m)[0].words() for m in moves], dtype=np.uint64).reshape(-1, 6)
     counts, _ = dev.perft_batch(kids, depth - 1, slider, True, 0, tt)
     return {m: int(c) for m, c in zip(moves, counts)}
+
+
+def perft_deep(dev, pos, depth, launch_depth=10, divided_depth=10, tt=None, slider=SLIDER_AUTO, on_divide=None):
+    """perft_bb_cpu_launcher + dividedPerft (launcher.h:1181-1450): perft of any
+    depth with a 128-bit total.  Nodes deeper than launch_depth are walked on
+    the host (with a host table of their totals), each node at launch_depth is
+    one device call (TT perft if tt is given).  on_divide(path, count) is
+    called for every move of a host node deeper than divided_depth, in the
+    reference's print order.  Returns (total, stats)."""
+    if isinstance(pos, str):
+        pos = read_fen_string(pos)
+    out = (u64 * 2)()
+    st = DeepStats()
+    cb = DIVIDE_CB(0)
+    if on_divide is not None:
+        def _cb(_user, path, lo, hi):
+            on_divide(path.decode(), int(lo) | (int(hi) << 64))
+        cb = DIVIDE_CB(_cb)
+    _check(lib().pg_perft_deep(dev._h, tt._h if tt is not None else None, ctypes.byref(pos), depth, launch_depth,
+                               divided_depth, slider, cb, None, out, ctypes.byref(st)), "pg_perft_deep")
+    return int(out[0]) | (int(out[1]) << 64), st.as_dict()
+
+
+def perft_records(dev, path, depth, tt=None, slider=SLIDER_AUTO, chunk=1024):
+    """processPerftRecords (launcher.h:1578-1695): perft(depth) of every FEN
+    line of `path` (last number = occurrence count) into path + ".op", resuming
+    after the lines the .op file already holds.  Returns (done, skipped)."""
+    done, skipped = ctypes.c_uint32(), ctypes.c_uint32()
+    _check(lib().pg_perft_records(dev._h, tt._h if tt is not None else None, os.fsencode(path), depth, slider, chunk,
+                                  ctypes.byref(done), ctypes.byref(skipped)), "pg_perft_records")
+    return int(done.value), int(skipped.value)
 
 
 def move_to_str(m):

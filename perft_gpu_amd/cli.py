@@ -1,14 +1,25 @@
 """Command line of the reference's perft.cu (perft.cu:14-164):
 
     python -m perft_gpu_amd.cli <fen> <depth> [numGPUs] [launchDepth] [--tt GB] [--slider N]
+    python -m perft_gpu_amd.cli --records <file> [--depth D] [--tt GB]
 
-Runs perft for every depth from 3 (minDepth, perft.cu:72) up to <depth> and
-prints the reference's result line for each (launcher.h:1496-1498).  argv[3]
-is the GPU count (the reference's usage text calls it launchdepth,
-perft.cu:82); only one GPU is used here.  argv[4], the launch depth, is
-accepted and ignored: the level loop sizes its own splits.  With --tt the
-transposition-table path runs (the reference's default build,
-switches.h:39).
+Runs perft for every depth from 3 (minDepth, perft.cu:72) up to <depth>.
+
+Without a table each depth is one GPU perft and prints the reference's
+no-TT line (launcher.h:1496-1498).  With --tt (the reference's default
+build, switches.h:39) each depth is a divided perft (dividedPerft,
+launcher.h:1392-1450): nodes deeper than the launch depth (argv[4], default
+10) are walked on the host with 128-bit totals, every move of a node deeper
+than 10 (DIVIDED_PERFT_DEPTH) prints "<path>   <count>", and the depth ends
+with "Perft(NN): <total>, time: .. s, gpuTime: .. s" (launcher.h:1439).
+
+argv[3] is the GPU count (the reference's usage text calls it launchdepth,
+perft.cu:82); only one GPU is used here.
+
+--records runs the reference's records mode (processPerftRecords,
+launcher.h:1578-1695): perft of every FEN line of <file> (its last number
+is an occurrence count) into <file>.op, resuming after the lines already
+there; the reference hard-codes depth 7.
 """
 import argparse
 import os
@@ -16,6 +27,9 @@ import sys
 import time
 
 import perft_gpu_amd as pg
+
+DIVIDED_PERFT_DEPTH = 10
+DEFAULT_LAUNCH_DEPTH = 10
 
 
 def main(argv=None):
@@ -30,7 +44,11 @@ def main(argv=None):
     ap.add_argument("--divide", action="store_true", help="print perft(depth-1) per root move at the last depth")
     ap.add_argument("--tt-file", default=None,
                     help="with --tt: resume the table from this checkpoint if it exists, and save it at the end")
+    ap.add_argument("--records", default=None, help="records mode: FEN file (last number of a line = count)")
+    ap.add_argument("--records-depth", type=int, default=7)
     args = ap.parse_args(argv)
+    if args.records:
+        return records(args)
     if args.fen is None or len(args.fen) <= 5:
         print("\nUsage perft_gpu_amd.cli <fen> <depth> [<numGPUs>] [<launchdepth>]")
         print("\nAs no paramaters were provided... running default test")
@@ -42,6 +60,7 @@ def main(argv=None):
     pos = pg.read_fen_string(fen)
     print("No of GPUs detected: 1")
     print(pg.pos_to_fen(pos))
+    launch = args.launch_depth if args.launch_depth else DEFAULT_LAUNCH_DEPTH
     with pg.Device(0) as dev:
         tt = pg.TranspositionTable(dev, int(args.tt * (1 << 30))) if args.tt > 0 else None
         if tt and args.tt_file and os.path.exists(args.tt_file):
@@ -49,8 +68,11 @@ def main(argv=None):
             print("resumed the transposition table from %s" % args.tt_file)
         try:
             for d in range(min(args.min_depth, args.depth), args.depth + 1):
+                if tt:
+                    divided_perft(dev, tt, pos, d, launch, args.slider)
+                    continue
                 t0 = time.perf_counter()
-                res = tt.perft(pos, d, args.slider) if tt else dev.perft(pos, d, args.slider)
+                res = dev.perft(pos, d, args.slider)
                 dt = time.perf_counter() - t0
                 print("\nGPU Perft %d: %d,   Time taken: %g seconds, nps: %d" % (d, res, dt, int(res / dt) if dt else 0))
                 sys.stdout.flush()
@@ -65,6 +87,37 @@ def main(argv=None):
         finally:
             if tt:
                 tt.close()
+    return 0
+
+
+def divided_perft(dev, tt, pos, depth, launch, slider):
+    """dividedPerft (launcher.h:1392-1450): the divided lines of every host
+    node deeper than DIVIDED_PERFT_DEPTH, then the total with wall and GPU time."""
+    def line(path, count):
+        print("%s   %20s" % (path, count))
+        sys.stdout.flush()
+
+    print()
+    t0 = time.perf_counter()
+    total, st = pg.perft_deep(dev, pos, depth, min(launch, 15), DIVIDED_PERFT_DEPTH, tt, slider, line)
+    dt = time.perf_counter() - t0
+    print("Perft(%02d):%20s, time: %8g s, gpuTime: %8g s" % (depth, total, dt, st["device_ms"] / 1e3))
+    sys.stdout.flush()
+    return total
+
+
+def records(args):
+    print("filename of op: %s.op" % args.records)
+    with pg.Device(0) as dev:
+        tt = pg.TranspositionTable(dev, int(args.tt * (1 << 30))) if args.tt > 0 else None
+        try:
+            t0 = time.perf_counter()
+            done, skipped = pg.perft_records(dev, args.records, args.records_depth, tt, args.slider)
+            dt = time.perf_counter() - t0
+        finally:
+            if tt:
+                tt.close()
+    print("\nRecords done: %d (skipped %d already in the output), Total: %g seconds" % (done, skipped, dt))
     return 0
 
 
