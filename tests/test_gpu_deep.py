@@ -49,8 +49,13 @@ def test_deep_with_table_and_host_memo(dev, pg):
         assert len(lines) == 48 and any("*" in p for p, _ in lines)
         print("deep start d9, launch depth 6", flush=True)
         total, st = pg.perft_deep(dev, W.CPW["start"], 9, 6, 8, tt)
-        assert total == W.CPW_COUNTS["start"][9]
-        assert st["host_tt_hits"] > 0          # e.g. 1.e3 e6 2.e4 and 1.e4 e6 2.e3
+        assert total == W.CPW_COUNTS["start"][9] and st["device_calls"] == 8902
+    # transpositions start at ply 3 (1.Rb3 Rh6 2.e3 = 1.e3 Rh6 2.Rb3): with the
+    # launch depth at 1 the ply-3 nodes of position 3 are host nodes
+    print("deep pos3 d5, launch depth 1", flush=True)
+    total, st = pg.perft_deep(dev, W.CPW["pos3"], 5, 1, 10)
+    assert total == W.CPW_COUNTS["pos3"][5]
+    assert st["host_tt_hits"] > 0 and st["host_nodes"] == 1 + 14 + 191 + 2812
     total, st = pg.perft_deep(dev, W.CPW["pos3"], 5, 10, 10)   # depth <= launch depth: one device call
     assert total == W.CPW_COUNTS["pos3"][5] and st["device_calls"] == 1 and st["host_nodes"] == 0
 
