@@ -119,6 +119,17 @@ int pg_set_slider(pg_ctx *ctx, int slider);
  * perft_bb_driver_gpu / perft_bb_gpu_simple (perft_bb.h:1027-1220):
  * breadth-first perft of one position; returns the leaf count. */
 int pg_perft(pg_ctx *ctx, const pg_pos48 *pos, uint32_t depth, int slider, uint64_t *out);
+/* No counterpart in the reference.  pg_perft for callers that cannot
+ * allocate device memory (code under HIP graph capture, tight serving loops,
+ * harnesses that forbid hipMalloc): the same breadth-first search and
+ * kernels, with the tree arena and the call's totals and flags in a static
+ * device array of 64 MiB per GPU (held until the process ends) instead of the
+ * context's arena.  Calls no hipMalloc, hipFree or other allocating routine;
+ * levels split to fit the 64 MiB, and a board whose children alone do not fit
+ * returns PG_ENOMEM with an error text.  All calls on one GPU share that
+ * array: calls that overlap on one GPU race (use one caller thread per GPU).
+ * Depth at most 15. */
+int pg_perft_noalloc(pg_ctx *ctx, const pg_pos48 *pos, uint32_t depth, int slider, uint64_t *out);
 int pg_last_stats(pg_ctx *ctx, pg_stats *out);
 /* Test hook (no reference counterpart; the reference sums deep totals in
  * InfInt, launcher.h:1181): every later perft call of this context starts its

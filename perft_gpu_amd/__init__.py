@@ -124,6 +124,7 @@ def lib():
         "pg_get_config": (ctypes.c_int, [ctypes.c_void_p, P(Config)]),
         "pg_set_slider": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int]),
         "pg_perft": (ctypes.c_int, [ctypes.c_void_p, P(Pos48), ctypes.c_uint32, ctypes.c_int, P(u64)]),
+        "pg_perft_noalloc": (ctypes.c_int, [ctypes.c_void_p, P(Pos48), ctypes.c_uint32, ctypes.c_int, P(u64)]),
         "pg_last_stats": (ctypes.c_int, [ctypes.c_void_p, P(Stats)]),
         "pg_test_count_bias": (ctypes.c_int, [ctypes.c_void_p, u64]),
         "pg_last_error": (ctypes.c_char_p, []),

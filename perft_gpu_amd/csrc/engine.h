@@ -8,6 +8,7 @@
 namespace pg {
 
 void set_error(const char *fmt, ...);
+int scratch_setup(int device, u8 **out);  // device address of the static scratch
 
 #define HIP_TRY(expr)                                                   \
     do {                                                                \
@@ -49,7 +50,8 @@ struct Engine {
     u32 *dErr = nullptr;
     u32 *dOvf = nullptr;     // set when a 64-bit total wraps
     u64 bias = 0;            // test hook: starting value of every root total
-    u64 *hostBuf = nullptr;  // pinned
+    u64 *hostBuf = nullptr;  // pinned, 256 bytes: [0] results, [1] flags, [8..13] a staged root
+    u8 *scratch = nullptr;   // device address of the static scratch (pg_perft_noalloc)
     pg_stats stats{};
 
     void *alloc(size_t bytes);

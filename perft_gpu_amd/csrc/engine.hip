@@ -13,6 +13,7 @@
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 #include <vector>
 #include "../../include/perft_gpu.h"
 #include "engine.h"
@@ -209,8 +210,9 @@ int Engine::bfs(int v, const Pos *boards, u64 n, const u32 *cnt, int r, int ply)
 // consistency check failed (the stream has been synchronized by the caller's
 // read of the result).
 int Engine::check_flags(const char *what) {
-    u32 f[2] = {0, 0};
-    HIP_TRY(hipMemcpy(f, dErr, sizeof(f), hipMemcpyDeviceToHost));
+    u32 *f = reinterpret_cast<u32 *>(hostBuf + 1);  // pinned staging: err, ovf
+    HIP_TRY(hipMemcpyAsync(f, dErr, 2 * sizeof(u32), hipMemcpyDeviceToHost, stream));
+    HIP_TRY(hipStreamSynchronize(stream));
     if (f[1]) {
         set_error("%s: a perft total does not fit 64 bits", what);
         return PG_EOVERFLOW;
@@ -242,7 +244,9 @@ int Engine::perft(const Pos &root, u32 depth, int v, u64 *out) {
     if (depth == 0) { *out = 1; return PG_OK; }
     Pos *d = (Pos *)alloc(sizeof(Pos));
     if (!d) return PG_ENOMEM;
-    HIP_TRY(hipMemcpyAsync(d, &root, sizeof(Pos), hipMemcpyHostToDevice, stream));
+    Pos *staged = reinterpret_cast<Pos *>(hostBuf + 8);  // pinned, so the copy needs no bounce buffer
+    *staged = root;
+    HIP_TRY(hipMemcpyAsync(d, staged, sizeof(Pos), hipMemcpyHostToDevice, stream));
     k_init_call<<<1, 64, 0, stream>>>(dAcc, bias, dItems, dErr, dOvf);
     int rc = bfs(v, d, 1, nullptr, (int)depth, 0);
     if (rc) return rc;
@@ -265,6 +269,40 @@ int Engine::perft(const Pos &root, u32 depth, int v, u64 *out) {
     return PG_OK;
 }
 
+}  // namespace pg
+
+// ============================================================ static scratch
+// Device memory for pg_perft_noalloc, a variant of pg_perft for callers that
+// cannot allocate (code under HIP graph capture, tight serving loops): the
+// tree arena and the per-call totals and flags of that entry point live in
+// this file-scope array instead of the context's arena.  Every GPU the library
+// runs on holds its own copy until the process ends.  In host code `scratch`
+// names a host-side stand-in, never the device memory: its device address is
+// taken once per device with hipGetSymbolAddress (scratch_setup, called from
+// pg_init with the device current) and only that address is used.
+namespace pg {
+constexpr size_t NOALLOC_SCRATCH_BYTES = 64ull << 20;  // 256 B of totals/flags, the rest tree arena
+constexpr size_t NOALLOC_HEAD = 256;
+static __device__ __align__(256) u8 scratch[NOALLOC_SCRATCH_BYTES];
+
+static std::mutex g_scratch_mu;
+static u8 *g_scratch_ptr[64];  // per device index
+
+int scratch_setup(int device, u8 **out) {
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (device < 0 || device >= 64) return PG_EINVAL;
+    if (!g_scratch_ptr[device]) {
+        void *p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !p) {
+            set_error("static scratch of device %d: hipGetSymbolAddress failed (%s)", device, hipGetErrorString(e));
+            return PG_EHIP;
+        }
+        g_scratch_ptr[device] = static_cast<u8 *>(p);
+    }
+    *out = g_scratch_ptr[device];
+    return PG_OK;
+}
 }  // namespace pg
 
 // ====================================================================== C ABI
@@ -330,7 +368,11 @@ int pg_init(int device, const pg_config *cfg, pg_ctx **out) {
     c->leafMode = c->cfg.leaf_mode;
     c->dErr = (u32 *)(c->dSmall + 64);
     c->dOvf = (u32 *)(c->dSmall + 68);
-    if (hipHostMalloc(&c->hostBuf, 64) != hipSuccess) return fail(PG_ENOMEM);
+    if (hipHostMalloc(&c->hostBuf, 256) != hipSuccess) return fail(PG_ENOMEM);
+    {
+        const int rc = scratch_setup(device, &c->scratch);
+        if (rc) return fail(rc);
+    }
     size_t freeB = 0, totalB = 0;
     (void)hipMemGetInfo(&freeB, &totalB);
     size_t want = c->cfg.arena_bytes ? c->cfg.arena_bytes : (48ull << 30);
@@ -388,6 +430,55 @@ int pg_perft(pg_ctx *c, const pg_pos48 *pos, uint32_t depth, int slider, uint64_
         return PG_EINVAL;
     }
     return c->perft(p, depth, v, reinterpret_cast<u64 *>(out));
+}
+
+int pg_perft_noalloc(pg_ctx *c, const pg_pos48 *pos, uint32_t depth, int slider, uint64_t *out) {
+    if (!c || !pos || !out) return PG_EINVAL;
+    if (depth > 15) {
+        set_error("pg_perft_noalloc: depth %u exceeds the maximum of 15", depth);
+        return PG_EINVAL;
+    }
+    const int v = resolve_slider(c, slider);
+    if (v < 0) return PG_EINVAL;
+    if (!c->scratch) {
+        set_error("pg_perft_noalloc: no static scratch on device %d", c->device);
+        return PG_ENODEV;
+    }
+    if (hipSetDevice(c->device) != hipSuccess) return PG_EHIP;
+    Pos p;
+    memcpy(&p, pos, sizeof(p));
+    if (!pos_sane(p)) {
+        set_error("position needs exactly one king per side");
+        return PG_EINVAL;
+    }
+    // run the plain search with its arena and per-call words in the static
+    // scratch, then give the context its own buffers back
+    u8 *const arena0 = c->arena;
+    const size_t bytes0 = c->arenaBytes, top0 = c->top, peak0 = c->peak;
+    u64 *const acc0 = c->dAcc, *const items0 = c->dItems;
+    u32 *const err0 = c->dErr, *const ovf0 = c->dOvf;
+    const int leafMode0 = c->leafMode;
+    c->dAcc = reinterpret_cast<u64 *>(c->scratch);
+    c->dItems = reinterpret_cast<u64 *>(c->scratch + 16);
+    c->dErr = reinterpret_cast<u32 *>(c->scratch + 64);
+    c->dOvf = reinterpret_cast<u32 *>(c->scratch + 68);
+    c->arena = c->scratch + NOALLOC_HEAD;
+    c->arenaBytes = NOALLOC_SCRATCH_BYTES - NOALLOC_HEAD;
+    c->leafMode = 0;
+    const int rc = c->perft(p, depth, v, reinterpret_cast<u64 *>(out));
+    c->arena = arena0;
+    c->arenaBytes = bytes0;
+    c->top = top0;
+    c->peak = peak0;
+    c->dAcc = acc0;
+    c->dItems = items0;
+    c->dErr = err0;
+    c->dOvf = ovf0;
+    c->leafMode = leafMode0;
+    if (rc == PG_ENOMEM)
+        set_error("pg_perft_noalloc: one board's children do not fit the %zu MiB static scratch",
+                  NOALLOC_SCRATCH_BYTES >> 20);
+    return rc;
 }
 
 int pg_test_count_bias(pg_ctx *c, uint64_t bias) {
