@@ -92,9 +92,20 @@ static __global__ __launch_bounds__(256) void kd_root_store(Plan *P, TTView tt, 
     }
 }
 
-// allocate the offsets and scan tiles of level l
+// release the transient arrays of level l (after its children are made)
+__device__ __forceinline__ void release_level(Plan *P, int l) {
+    DLevel &L = P->lv[l];
+    if (P->lv[l + 1].n) P->top = L.release;
+    L.mv = nullptr;
+    L.par = nullptr;
+    L.set = nullptr;
+}
+
+// release level l-1's transient arrays (l > 0), then allocate the offsets
+// and scan tiles of level l
 static __global__ void kd_plan_scan(Plan *P, int l) {
     if (threadIdx.x || blockIdx.x) return;
+    if (l > 0) release_level(P, l - 1);
     DLevel &L = P->lv[l];
     L.off = nullptr;
     L.tiles = nullptr;
@@ -120,20 +131,25 @@ static __global__ __launch_bounds__(SCAN_WG) void kd_scan_reduce(Plan *P, int l)
     }
 }
 
-static __global__ __launch_bounds__(1024) void kd_scan_spine(Plan *P, int l) {
+__device__ void plan_children(Plan *P, int l, int wantKeys, int wantSet, int total);
+
+// scan of the tile sums; then one thread plans level l+1 from the total
+static __global__ __launch_bounds__(1024) void kd_scan_spine(Plan *P, int l, int wantKeys, int wantSet, int total) {
     const DLevel L = P->lv[l];
-    if (L.n == 0) return;
-    const u64 tiles = (L.n + SCAN_TILE - 1) / SCAN_TILE;
-    u64 carry = 0;
-    for (u64 s = 0; s < tiles; s += 1024) {
-        const u64 i = s + threadIdx.x;
-        const u64 v = i < tiles ? L.tiles[i] : 0;
-        u64 tot;
-        const u64 ex = block_excl_scan<1024>(v, &tot);
-        if (i < tiles) L.tiles[i] = carry + ex;
-        carry += tot;
+    if (L.n) {
+        const u64 tiles = (L.n + SCAN_TILE - 1) / SCAN_TILE;
+        u64 carry = 0;
+        for (u64 s = 0; s < tiles; s += 1024) {
+            const u64 i = s + threadIdx.x;
+            const u64 v = i < tiles ? L.tiles[i] : 0;
+            u64 tot;
+            const u64 ex = block_excl_scan<1024>(v, &tot);
+            if (i < tiles) L.tiles[i] = carry + ex;
+            carry += tot;
+        }
+        if (threadIdx.x == 0) L.off[L.n] = carry;
     }
-    if (threadIdx.x == 0) L.off[L.n] = carry;
+    if (threadIdx.x == 0) plan_children(P, l, wantKeys, wantSet, total);
 }
 
 static __global__ __launch_bounds__(SCAN_WG) void kd_scan_down(Plan *P, int l) {
@@ -156,9 +172,9 @@ static __global__ __launch_bounds__(SCAN_WG) void kd_scan_down(Plan *P, int l) {
     }
 }
 
-// allocate level l+1 (children of l) and the transient arrays
-static __global__ void kd_plan_children(Plan *P, int l, int wantKeys, int wantSet) {
-    if (threadIdx.x || blockIdx.x) return;
+// allocate level l+1 (children of l) and the transient arrays; a total-only
+// search keeps just boards and counts per level
+__device__ void plan_children(Plan *P, int l, int wantKeys, int wantSet, int total) {
     DLevel &L = P->lv[l];
     DLevel &C = P->lv[l + 1];
     memset(&C, 0, sizeof(C));
@@ -169,9 +185,11 @@ static __global__ void kd_plan_children(Plan *P, int l, int wantKeys, int wantSe
     }
     C.b = (Pos *)plan_alloc(P, m * sizeof(Pos));
     C.cnt = (u32 *)plan_alloc(P, m * 4);
-    C.perft = (u64 *)plan_alloc(P, m * 8);
-    C.state = (u8 *)plan_alloc(P, m);
-    C.dup = (u32 *)plan_alloc(P, m * 4);
+    if (!total) {
+        C.perft = (u64 *)plan_alloc(P, m * 8);
+        C.state = (u8 *)plan_alloc(P, m);
+        C.dup = (u32 *)plan_alloc(P, m * 4);
+    }
     if (wantKeys) C.k = (Key128 *)plan_alloc(P, m * sizeof(Key128));
     L.release = P->top;
     L.mv = (u16 *)plan_alloc(P, m * 2);
@@ -189,27 +207,22 @@ static __global__ void kd_plan_children(Plan *P, int l, int wantKeys, int wantSe
     C.n = m;
 }
 
-static __global__ void kd_release(Plan *P, int l) {
-    if (threadIdx.x || blockIdx.x) return;
-    DLevel &L = P->lv[l];
-    if (P->lv[l + 1].n) P->top = L.release;
-    L.mv = nullptr;
-    L.par = nullptr;
-    L.set = nullptr;
-}
-
 template <int V>
 __global__ __launch_bounds__(wg_size<V>()) void kd_generate(Plan *P, int l, Tables t) {
     stage_tables<V>(t);
     const DLevel L = P->lv[l];
     if (P->lv[l + 1].n == 0) return;
-    PG_GRID_LOOP(i, L.n) {
-        const u64 o = L.off[i];
-        const u32 room = (u32)(L.off[i + 1] - o);
-        if (room == 0) continue;
-        const u32 c = generate_any<V>(L.b[i], t, L.mv + o, room);
-        if (c != room) P->err = 1;
-        for (u32 j = 0; j < room; j++) L.par[o + j] = (u32)i;
+    if constexpr (V == SL_BYTE) {
+        PG_GRID_LOOP(i, L.n) {
+            const u64 o = L.off[i];
+            const u32 room = (u32)(L.off[i + 1] - o);
+            if (room == 0) continue;
+            const u32 c = generate_any<V>(L.b[i], t, L.mv + o, room);
+            if (c != room) P->err = 1;
+            for (u32 j = 0; j < room; j++) L.par[o + j] = (u32)i;
+        }
+    } else {
+        generate_tiles<V>(L.b, L.n, L.off, L.mv, L.par, &P->err, t);
     }
 }
 
@@ -237,16 +250,13 @@ __global__ __launch_bounds__(wg_size<V>()) void kd_expand(Plan *P, int l, TTView
         } else {
             make_any(p, L.mv[i], t);
         }
-        if (hit) {
-            C.cnt[i] = 0;
-            C.state[i] = ST_HIT;
-        } else {
-            C.cnt[i] = count_any<V>(p, t);
-            C.state[i] = ST_LIVE;
-        }
+        C.cnt[i] = hit ? 0 : count_any<V>(p, t);
         C.b[i] = p;
-        C.perft[i] = v;
-        C.dup[i] = (u32)i;
+        if (C.state) {  // absent in a total-only search
+            C.state[i] = hit ? ST_HIT : ST_LIVE;
+            C.perft[i] = v;
+            C.dup[i] = (u32)i;
+        }
     }
 }
 
@@ -302,60 +312,23 @@ static __global__ __launch_bounds__(256) void kd_dedup(Plan *P, int l, TTCounter
     }
 }
 
-// perft(2) of the boards of level l, fused (see k_leaf_fused)
-template <int V>
-__global__ __launch_bounds__(LEAF_WG) __attribute__((amdgpu_waves_per_eu(V == SL_KOGGE ? 4 : 5, 8))) void kd_leaf(Plan *P, int l, u32 *err, Tables t) {
+// perft(2) of the boards of level l, fused (see leaf_tiles): per board into
+// the level's perft array, or (TOTAL, a single-root search that only needs
+// the sum) added to the plan's total
+template <int V, bool TOTAL>
+__global__ __launch_bounds__(LEAF_WG) __attribute__((amdgpu_waves_per_eu(V == SL_KOGGE ? 4 : 5, 8))) void kd_leaf(Plan *P, int l, Tables t) {
     const DLevel L = P->lv[l];
-    __shared__ Pos sb[LEAF_WG];
-    constexpr int LEAF_CAP = leaf_cap<true>();
-    __shared__ u16 smv[LEAF_CAP];
-    __shared__ u16 stag[LEAF_CAP];  // owner | moving piece << 8
-    __shared__ u32 sper[LEAF_WG];
-    __shared__ u32 snext;
-    const int tid = threadIdx.x;
-    for (u64 tile = blockIdx.x; tile * LEAF_WG < L.n; tile += gridDim.x) {
-        const u64 i = tile * LEAF_WG + tid;
-        const bool live = i < L.n;
-        const u32 c = live ? L.cnt[i] : 0;
-        if (live) sb[tid] = L.b[i];
-        sper[tid] = 0;
-        u64 total64;
-        const u32 pos = (u32)block_excl_scan<LEAF_WG>(c, &total64);
-        const u32 total = (u32)total64;
-        if (tid == 0) atomicAdd(&P->leafItems, total64);
-        u32 start = 0;
-        while (start < total) {
-            const u32 end = start + LEAF_CAP;
-            if (tid == 0) snext = total;
-            __syncthreads();
-            const bool here = c && pos >= start && pos + c <= end;
-            if (here) {
-                const u32 g = generate_any<V>(sb[tid], t, smv + (pos - start), c, stag + (pos - start), (u32)tid);
-                if (g != c) *err = 2;
-            } else if (c && pos >= start) {
-                atomicMin(&snext, pos);
-            }
-            __syncthreads();
-            const u32 wend = snext;
-            for (u32 j = start + tid; j < wend; j += LEAF_WG) {
-                const u32 k = j - start;
-                const u32 tg = stag[k];
-                const int b = tg & 255;
-                atomicAdd(&sper[b], count_after_as<V>(sb[b], smv[k], (int)(tg >> 8), t));
-            }
-            __syncthreads();
-            start = wend;
-        }
-        if (live && L.state[i] == ST_LIVE) {
-            u64 v = sper[tid];
-            if (l == 0 && P->bias) {
-                v += P->bias;
-                if (v < P->bias) P->ovf = 1;
-            }
-            L.perft[i] = v;
-        }
-        __syncthreads();
-    }
+    leaf_tiles<V, !TOTAL>(L.b, L.cnt, L.n, &P->acc, &P->leafItems, L.perft, L.state, l == 0 ? P->bias : 0, &P->err,
+                          &P->ovf, t);
+}
+
+// the plan's total += the move counts of level l (perft(1) of a total-only search)
+static __global__ __launch_bounds__(256) void kd_sum_counts(Plan *P, int l) {
+    const DLevel L = P->lv[l];
+    u64 s = 0;
+    PG_GRID_LOOP(i, L.n) s += L.cnt[i];
+    s = wave_sum(s);
+    if ((threadIdx.x & 63) == 0 && s) add_checked(&P->acc, s, &P->ovf);
 }
 
 static __global__ __launch_bounds__(256) void kd_perft1(Plan *P, int l) {

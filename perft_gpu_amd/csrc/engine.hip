@@ -242,6 +242,22 @@ int Engine::perft(const Pos &root, u32 depth, int v, u64 *out) {
     top = 0;
     peak = 0;
     if (depth == 0) { *out = 1; return PG_OK; }
+    // Whole search as one graph launch (no host round trip between levels)
+    // unless each leaf launch is to be timed with events, the materialized
+    // leaf is asked for, or the call must not allocate (its plan and graphs
+    // are created on first use).  A search whose levels do not fit the arena
+    // falls back to the host-driven loop below, which splits levels.
+    if (!timing && leafMode == 0 && graphPerft) {
+        bool fits = false;
+        const int rc = perft_graph(*this, root, depth, v, out, &fits);
+        if (rc) return rc;
+        if (fits) {
+            stats.leaves = *out;
+            stats.total_ms = now_ms() - t0;
+            return PG_OK;
+        }
+        memset(&stats, 0, sizeof(stats));
+    }
     Pos *d = (Pos *)alloc(sizeof(Pos));
     if (!d) return PG_ENOMEM;
     Pos *staged = reinterpret_cast<Pos *>(hostBuf + 8);  // pinned, so the copy needs no bounce buffer
@@ -458,6 +474,7 @@ int pg_perft_noalloc(pg_ctx *c, const pg_pos48 *pos, uint32_t depth, int slider,
     u64 *const acc0 = c->dAcc, *const items0 = c->dItems;
     u32 *const err0 = c->dErr, *const ovf0 = c->dOvf;
     const int leafMode0 = c->leafMode;
+    c->graphPerft = false;  // the graph path's plan and graphs are allocated on first use
     c->dAcc = reinterpret_cast<u64 *>(c->scratch);
     c->dItems = reinterpret_cast<u64 *>(c->scratch + 16);
     c->dErr = reinterpret_cast<u32 *>(c->scratch + 64);
@@ -475,6 +492,7 @@ int pg_perft_noalloc(pg_ctx *c, const pg_pos48 *pos, uint32_t depth, int slider,
     c->dErr = err0;
     c->dOvf = ovf0;
     c->leafMode = leafMode0;
+    c->graphPerft = true;
     if (rc == PG_ENOMEM)
         set_error("pg_perft_noalloc: one board's children do not fit the %zu MiB static scratch",
                   NOALLOC_SCRATCH_BYTES >> 20);

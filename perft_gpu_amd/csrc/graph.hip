@@ -22,8 +22,10 @@ namespace pg {
 struct GraphShape {
     int depth, v, keys, dedup, bpc;
     const TTable *tt;
+    int total;  // 1: one root, only the sum is wanted (no per-board perft arrays)
     bool operator==(const GraphShape &o) const {
-        return depth == o.depth && v == o.v && keys == o.keys && dedup == o.dedup && bpc == o.bpc && tt == o.tt;
+        return depth == o.depth && v == o.v && keys == o.keys && dedup == o.dedup && bpc == o.bpc && tt == o.tt &&
+               total == o.total;
     }
 };
 
@@ -54,7 +56,32 @@ template <int V> static void cap_expand(hipStream_t s, Plan *P, int l, const TTV
     kd_expand<V><<<g, wg_size<V>(), 0, s>>>(P, l, tt, rc, ctr, t);
 }
 template <int V> static void cap_leaf(hipStream_t s, Plan *P, int l, int g, const Tables &t) {
-    if constexpr (V != SL_BYTE) kd_leaf<V><<<g, LEAF_WG, 0, s>>>(P, l, &P->err, t);
+    if constexpr (V != SL_BYTE) kd_leaf<V, false><<<g, LEAF_WG, 0, s>>>(P, l, t);
+}
+template <int V> static void cap_leaf_total(hipStream_t s, Plan *P, int l, int g, const Tables &t) {
+    if constexpr (V != SL_BYTE) kd_leaf<V, true><<<g, LEAF_WG, 0, s>>>(P, l, t);
+}
+
+// The search of one root when only its perft is wanted: per level just
+// boards and counts (no keys, table or per-board sums), the leaf adds into the
+// plan's total.
+static void enqueue_total(const Engine &e, hipStream_t s, Plan *P, const GraphShape &sh) {
+    const int D = sh.depth, v = sh.v;
+    const int gWork = e.cus * sh.bpc, gScan = e.cus * 4;
+    const int gv = v == SL_BYTE ? e.cus : gWork;
+    PG_VARIANT_CALL(v, cap_count, s, P, gv, e.dt);
+    const int lastExpand = v == SL_BYTE ? D - 2 : D - 3;
+    for (int l = 0; l <= lastExpand; l++) {
+        kd_plan_scan<<<1, 64, 0, s>>>(P, l);
+        kd_scan_reduce<<<gScan, SCAN_WG, 0, s>>>(P, l);
+        kd_scan_spine<<<1, 1024, 0, s>>>(P, l, 0, 0, 1);
+        kd_scan_down<<<gScan, SCAN_WG, 0, s>>>(P, l);
+        PG_VARIANT_CALL(v, cap_generate, s, P, l, gv, e.dt);
+        PG_VARIANT_CALL(v, cap_expand, s, P, l, TTView{}, D - l - 1, (TTCounters *)nullptr, gv, e.dt);
+    }
+    const int last = lastExpand + 1 < 0 ? 0 : lastExpand + 1;  // deepest materialized level
+    if (D == 1 || v == SL_BYTE) kd_sum_counts<<<gWork, 256, 0, s>>>(P, last);
+    else PG_VARIANT_CALL(v, cap_leaf_total, s, P, last, e.cus * LEAF_BLOCKS_PER_CU, e.dt);
 }
 
 // Enqueues the whole search for `depth` on stream s (captured by the caller).
@@ -75,19 +102,17 @@ static void enqueue_search(const Engine &e, hipStream_t s, Plan *P, const GraphS
     const int lastExpand = v == SL_BYTE ? D - 2 : D - 3;
     for (int l = 0; l <= lastExpand; l++) {
         const int rc = D - l - 1;
+        const int wantSet = sh.dedup && rc >= 2;
         kd_plan_scan<<<1, 64, 0, s>>>(P, l);
         kd_scan_reduce<<<gScan, SCAN_WG, 0, s>>>(P, l);
-        kd_scan_spine<<<1, 1024, 0, s>>>(P, l);
+        kd_scan_spine<<<1, 1024, 0, s>>>(P, l, sh.keys, wantSet, 0);
         kd_scan_down<<<gScan, SCAN_WG, 0, s>>>(P, l);
-        const int wantSet = sh.dedup && rc >= 2;
-        kd_plan_children<<<1, 64, 0, s>>>(P, l, sh.keys, wantSet);
         PG_VARIANT_CALL(v, cap_generate, s, P, l, gv, e.dt);
         PG_VARIANT_CALL(v, cap_expand, s, P, l, tt, rc, ctr, gv, e.dt);
         if (wantSet) {
             kd_clear_set<<<gWork, 256, 0, s>>>(P, l);
             kd_dedup<<<gWork, 256, 0, s>>>(P, l, ctr, rc);
         }
-        kd_release<<<1, 64, 0, s>>>(P, l);
     }
     const int last = lastExpand + 1;  // deepest materialized level
     if (D == 1 || v == SL_BYTE) {
@@ -110,7 +135,8 @@ static int get_graph(Engine &e, Slot &sl, const GraphShape &sh, hipGraphExec_t *
         }
     hipGraph_t graph;
     HIP_TRY(hipStreamBeginCapture(sl.stream, hipStreamCaptureModeThreadLocal));
-    enqueue_search(e, sl.stream, sl.dPlan, sh);
+    if (sh.total) enqueue_total(e, sl.stream, sl.dPlan, sh);
+    else enqueue_search(e, sl.stream, sl.dPlan, sh);
     HIP_TRY(hipStreamEndCapture(sl.stream, &graph));
     hipGraphExec_t exec;
     HIP_TRY(hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0));
@@ -220,6 +246,58 @@ void slot_forget_tt(Slot &sl, const TTable *tt) {
     }
 }
 
+// perft of one root through a total-only graph on the context's batch slot:
+// no host round trip between levels.  *fits is false (and nothing is
+// returned) when a level does not fit the arena; the caller then runs the
+// host-driven search, which splits levels.
+int perft_graph(Engine &e, const Pos &root, u32 depth, int v, u64 *out, bool *fits) {
+    *fits = false;
+    if (!e.batchSlot) {
+        e.batchSlot = new Slot();
+        const int rc = slot_init(e, *e.batchSlot, e.arena, e.arenaBytes, e.stream);
+        if (rc) return rc;
+    }
+    Slot &sl = *e.batchSlot;
+    Plan &h = *sl.hPlan;
+    memset(&h, 0, sizeof(Plan));
+    h.base = sl.region;
+    h.cap = sl.regionBytes;
+    // level 0: the root (staged through the pinned plan copy) and its count
+    h.lv[0].n = 1;
+    h.lv[0].b = (Pos *)sl.region;
+    h.lv[0].cnt = (u32 *)(sl.region + 256);
+    h.top = 512;
+    h.acc = e.bias;
+    GraphShape sh{(int)depth, v, 0, 0, e.bpc, nullptr, 1};
+    hipGraphExec_t exec;
+    int rc = get_graph(e, sl, sh, &exec);
+    if (rc) return rc;
+    HIP_TRY(hipMemcpyAsync(sl.dPlan, &h, sizeof(Plan), hipMemcpyHostToDevice, sl.stream));
+    Pos *staged = reinterpret_cast<Pos *>(e.hostBuf + 8);
+    *staged = root;
+    HIP_TRY(hipMemcpyAsync(h.lv[0].b, staged, sizeof(Pos), hipMemcpyHostToDevice, sl.stream));
+    HIP_TRY(hipGraphLaunch(exec, sl.stream));
+    HIP_TRY(hipMemcpyAsync(&h, sl.dPlan, sizeof(Plan), hipMemcpyDeviceToHost, sl.stream));
+    HIP_TRY(hipStreamSynchronize(sl.stream));
+    if (h.overflow) return PG_OK;
+    if (h.ovf) {
+        set_error("perft: a perft total does not fit 64 bits");
+        return PG_EOVERFLOW;
+    }
+    if (h.err) {
+        set_error("perft: device consistency check failed (flag %u)", h.err);
+        return PG_EINVAL;
+    }
+    *fits = true;
+    *out = h.acc;
+    e.stats.leaf_items = h.leafItems;
+    e.stats.leaf_launches = depth >= 2 && v != SL_BYTE ? 1 : 0;
+    for (u32 l = 1; l < depth && l < 16; l++) e.stats.nodes[l] = h.lv[l].n;
+    if (depth >= 2 && depth - 1 < 16 && v != SL_BYTE) e.stats.nodes[depth - 1] = h.leafItems;
+    if (depth >= 2 && v != SL_BYTE) e.stats.leaf_parents = h.lv[depth - 2].n;
+    return PG_OK;
+}
+
 // Batched perft of n roots on slot sl.  Results in out[] (host), in the
 // caller's order.
 int batch_perft(Engine &e, Slot &sl, TTable *tt, const Pos *roots, u64 n, u32 depth, int v, int dedup, int bpc,
@@ -266,7 +344,7 @@ int batch_perft(Engine &e, Slot &sl, TTable *tt, const Pos *roots, u64 n, u32 de
     }
     HIP_TRY(hipMemcpyAsync(L0.b, hb.data(), n * sizeof(Pos), hipMemcpyHostToDevice, sl.stream));
     if (keys) HIP_TRY(hipMemcpyAsync(L0.k, hk.data(), n * sizeof(Key128), hipMemcpyHostToDevice, sl.stream));
-    GraphShape sh{(int)depth, v, keys ? 1 : 0, dedup ? 1 : 0, bpc > 0 ? bpc : e.bpc, tt};
+    GraphShape sh{(int)depth, v, keys ? 1 : 0, dedup ? 1 : 0, bpc > 0 ? bpc : e.bpc, tt, 0};
     u64 leafItems = 0, launches = 0, splits = 0;
     double ms = 0;
     int rc = run_split(e, sl, sh, 0, n, L0, top, &leafItems, &ms, &launches, &splits);
@@ -463,7 +541,7 @@ int pg_server_submit(pg_server *s, uint32_t slot, const pg_pos48 *pos, uint32_t 
     if (!L0.dup) return PG_ENOMEM;
     HIP_TRY(hipMemcpyAsync(L0.b, hb.data(), n * sizeof(Pos), hipMemcpyHostToDevice, sl.stream));
     HIP_TRY(hipMemcpyAsync(L0.k, hk.data(), n * sizeof(Key128), hipMemcpyHostToDevice, sl.stream));
-    GraphShape sh{(int)depth, v, 1, dedup ? 1 : 0, e.bpc, s->tt};
+    GraphShape sh{(int)depth, v, 1, dedup ? 1 : 0, e.bpc, s->tt, 0};
     hipGraphExec_t exec;
     int rc = get_graph(e, sl, sh, &exec);
     if (rc) return rc;
@@ -515,7 +593,7 @@ int pg_server_poll(pg_server *s, uint32_t slot, uint64_t *out, pg_batch_stats *s
     u64 items = h.leafItems, launches = 1, splits = 0;
     double ms = 0;
     if (h.overflow) {
-        GraphShape sh{(int)ss.depth, ss.v, 1, ss.dedup ? 1 : 0, e.bpc, s->tt};
+        GraphShape sh{(int)ss.depth, ss.v, 1, ss.dedup ? 1 : 0, e.bpc, s->tt, 0};
         (void)hipSetDevice(e.device);
         items = 0;
         launches = 0;

@@ -153,20 +153,73 @@ static __global__ __launch_bounds__(256) void k_sum_u32(const u32 *__restrict__ 
     if ((threadIdx.x & 63) == 0 && s) add_checked(acc, s, ovf);
 }
 
-// err: set to 1 if a board generates a different number of moves than it
-// counted (the two must agree for the level layout to be valid).
+// Move lists and parent indices of boards [0, n) whose exclusive child
+// offsets are off[0..n].  A work-group takes 256 consecutive boards, whose
+// children form one contiguous output range: the boards generate into an LDS
+// window (move, and owner | piece tag), then the work-group stores the window
+// one lane per slot, so the HBM writes are coalesced and each byte is written
+// once.  Boards that do not fit the window wait for the next pass.  err is
+// set to 1 if a board generates a different number of moves than it counted
+// (the two must agree for the level layout to be valid).
+constexpr int GEN_WG = 256, GEN_WINDOW = 8192;
+
+template <int V>
+__device__ __forceinline__ void generate_tiles(const Pos *__restrict__ b, u64 n, const u64 *__restrict__ off,
+                                               u16 *__restrict__ moves, u32 *__restrict__ parent, u32 *err,
+                                               const Tables &t) {
+    __shared__ u16 smv[GEN_WINDOW];
+    __shared__ u16 stag[GEN_WINDOW];
+    __shared__ u32 snext;
+    const int tid = threadIdx.x;
+    for (u64 tile = blockIdx.x; tile * GEN_WG < n; tile += gridDim.x) {
+        const u64 first = tile * GEN_WG;
+        const u64 last = first + GEN_WG < n ? first + GEN_WG : n;
+        const u64 i = first + tid;
+        const bool live = i < n;
+        const u64 base = off[first];
+        const u32 pos = live ? (u32)(off[i] - base) : 0;
+        const u32 c = live ? (u32)(off[i + 1] - off[i]) : 0;  // 0: no moves, or resolved earlier
+        const u32 total = (u32)(off[last] - base);
+        u32 start = 0;
+        while (start < total) {
+            const u32 end = start + GEN_WINDOW;
+            if (tid == 0) snext = total;
+            __syncthreads();
+            if (c && pos >= start && pos + c <= end) {
+                const u32 g = generate_any<V>(b[i], t, smv + (pos - start), c, stag + (pos - start), (u32)tid);
+                if (g != c) *err = 1;
+            } else if (c && pos >= start) {
+                atomicMin(&snext, pos);
+            }
+            __syncthreads();
+            const u32 wend = snext;
+            for (u32 j = start + tid; j < wend; j += GEN_WG) {
+                moves[base + j] = smv[j - start];
+                parent[base + j] = (u32)first + (stag[j - start] & 255u);
+            }
+            __syncthreads();
+            start = wend;
+        }
+    }
+}
+
 template <int V>
 __global__ __launch_bounds__(wg_size<V>()) void k_generate(const Pos *__restrict__ b, u64 n, const u64 *__restrict__ off,
                                                            u16 *__restrict__ moves, u32 *__restrict__ parent, u32 *err,
                                                            Tables t) {
     stage_tables<V>(t);
-    PG_GRID_LOOP(i, n) {
-        const u64 o = off[i];
-        const u32 room = (u32)(off[i + 1] - o);
-        if (room == 0) continue;  // no moves, or a board resolved earlier
-        const u32 c = generate_any<V>(b[i], t, moves + o, room);
-        if (c != room) *err = 1;
-        for (u32 j = 0; j < room; j++) parent[o + j] = (u32)i;
+    if constexpr (V == SL_BYTE) {
+        // the byte-lookup tables fill the LDS: each lane writes its own list
+        PG_GRID_LOOP(i, n) {
+            const u64 o = off[i];
+            const u32 room = (u32)(off[i + 1] - o);
+            if (room == 0) continue;
+            const u32 c = generate_any<V>(b[i], t, moves + o, room);
+            if (c != room) *err = 1;
+            for (u32 j = 0; j < room; j++) parent[o + j] = (u32)i;
+        }
+    } else {
+        generate_tiles<V>(b, n, off, moves, parent, err, t);
     }
 }
 
@@ -239,9 +292,16 @@ __device__ __forceinline__ u64 block_excl_scan(u64 v, u64 *total) {
 constexpr int LEAF_WG = 256, LEAF_BLOCKS_PER_CU = 64;
 template <bool PER> constexpr int leaf_cap() { return PER ? 6848 : 7040; }
 
+// state of a board after expansion (TT and batch searches)
+enum : u8 { ST_LIVE = 0, ST_HIT = 1, ST_DUP = 2 };
+
+// The fused leaf over boards [0, n) with move counts cnt[]: PER = false adds
+// every count to *acc; PER = true writes base + perft(2) to per[i] for live
+// boards (state[i] == ST_LIVE when state is given, else a valid board).
 template <int V, bool PER>
-__global__ __launch_bounds__(LEAF_WG) __attribute__((amdgpu_waves_per_eu(V == SL_KOGGE ? 4 : 5, 8))) void k_leaf_fused(const Pos *__restrict__ boards, const u32 *__restrict__ cnt, u64 n,
-                                                        u64 *acc, u64 *items, u64 *__restrict__ per, u32 *err, u32 *ovf, Tables t) {
+__device__ __forceinline__ void leaf_tiles(const Pos *__restrict__ boards, const u32 *__restrict__ cnt, u64 n, u64 *acc,
+                                           u64 *items, u64 *__restrict__ per, const u8 *__restrict__ state, u64 base,
+                                           u32 *err, u32 *ovf, const Tables &t) {
     static_assert(V != SL_BYTE, "the byte-lookup tables leave no LDS for the move window");
     __shared__ Pos sb[LEAF_WG];
     constexpr int LEAF_CAP = leaf_cap<PER>();
@@ -250,6 +310,7 @@ __global__ __launch_bounds__(LEAF_WG) __attribute__((amdgpu_waves_per_eu(V == SL
     __shared__ u32 sper[PER ? LEAF_WG : 1];
     __shared__ u32 snext;
     const int tid = threadIdx.x;
+    const Tables &tl = t;
     u64 s = 0;
     for (u64 tile = blockIdx.x; tile * LEAF_WG < n; tile += gridDim.x) {
         const u64 i = tile * LEAF_WG + tid;
@@ -268,7 +329,7 @@ __global__ __launch_bounds__(LEAF_WG) __attribute__((amdgpu_waves_per_eu(V == SL
             __syncthreads();
             const bool here = c && pos >= start && pos + c <= end;
             if (here) {
-                const u32 g = generate_any<V>(sb[tid], t, smv + (pos - start), c, stag + (pos - start), (u32)tid);
+                const u32 g = generate_any<V>(sb[tid], tl, smv + (pos - start), c, stag + (pos - start), (u32)tid);
                 if (g != c) *err = 2;
             } else if (c && pos >= start) {
                 atomicMin(&snext, pos);
@@ -279,20 +340,31 @@ __global__ __launch_bounds__(LEAF_WG) __attribute__((amdgpu_waves_per_eu(V == SL
                 const u32 k = j - start;
                 const u32 tg = stag[k];
                 const int b = tg & 255;
-                const u32 m = count_after_as<V>(sb[b], smv[k], (int)(tg >> 8), t);
+                const u32 m = count_after_as<V>(sb[b], smv[k], (int)(tg >> 8), tl);
                 if (PER) atomicAdd(&sper[b], m);
                 else s += m;
             }
             __syncthreads();
             start = wend;
         }
-        if (PER && live && pos_valid(sb[tid])) per[i] = sper[tid];
+        if (PER && live && (state ? state[i] == ST_LIVE : pos_valid(sb[tid]))) {
+            const u64 v = base + sper[tid];
+            if (v < base) *ovf = 1;
+            per[i] = v;
+        }
         __syncthreads();
     }
     if (!PER) {
         s = wave_sum(s);
         if ((tid & 63) == 0 && s) add_checked(acc, s, ovf);
     }
+}
+
+template <int V, bool PER>
+__global__ __launch_bounds__(LEAF_WG) __attribute__((amdgpu_waves_per_eu(V == SL_KOGGE ? 4 : 5, 8))) void k_leaf_fused(
+    const Pos *__restrict__ boards, const u32 *__restrict__ cnt, u64 n, u64 *acc, u64 *items, u64 *__restrict__ per,
+    u32 *err, u32 *ovf, Tables t) {
+    leaf_tiles<V, PER>(boards, cnt, n, acc, items, per, nullptr, 0, err, ovf, t);
 }
 
 // ---------------------------------------------------------------- scan

@@ -73,8 +73,6 @@ __device__ __forceinline__ bool tt_store(const TTView &tt, int r, const Key128 &
     return true;
 }
 
-// child state after expansion
-enum : u8 { ST_LIVE = 0, ST_HIT = 1, ST_DUP = 2 };
 
 // counters: [r][0] probes, [r][1] hits, [r][2] stores, [r][3] duplicates
 struct TTCounters {

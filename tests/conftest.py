@@ -38,6 +38,15 @@ def pg():
 
 @pytest.fixture(scope="session")
 def dev(pg):
+    """Context that times every leaf launch: pg_perft runs the host-driven level loop."""
     d = pg.Device(0, arena_bytes=24 << 30, timing=True)
+    yield d
+    d.close()
+
+
+@pytest.fixture(scope="session")
+def gdev(pg):
+    """Context without per-launch timing: pg_perft runs as one graph launch."""
+    d = pg.Device(0, arena_bytes=16 << 30)
     yield d
     d.close()

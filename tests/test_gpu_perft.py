@@ -53,7 +53,8 @@ def test_start_depth8(dev, v):
 
 def test_small_arena_forces_splits_and_stays_exact(pg):
     # perft(7) needs ~250 MB for ply 5 and ~700 MB for the ply-6 items; a
-    # 64 MiB arena forces both levels to split
+    # 64 MiB arena forces both levels to split (the graph path finds that its
+    # levels do not fit and hands over to the host-driven loop)
     with pg.Device(0, arena_bytes=64 << 20) as small:
         for v in (0, 3):
             print("split arena start d7 variant", v, flush=True)
@@ -135,3 +136,47 @@ def test_c_caller_of_the_abi(pg):
     out = subprocess.run([exe, W.CPW["kiwipete"], "4", "1"], capture_output=True, text=True, timeout=120)
     assert out.returncode == 0, out.stderr
     assert "GPU Perft 4: %d," % W.CPW_COUNTS["kiwipete"][4] in out.stdout
+
+
+@pytest.mark.parametrize("v", VARIANTS)
+def test_graph_path_start_position_and_node_counts(gdev, v):
+    """pg_perft as one graph launch: start perft(0..8) and the nodes per ply."""
+    for d in range(0, 9):
+        print("graph start d%d variant %d" % (d, v), flush=True)
+        assert gdev.perft(W.CPW["start"], d, v) == W.CPW_COUNTS["start"][d]
+    st = gdev.last_stats()
+    assert st["nodes"][1:8] == W.CPW_COUNTS["start"][1:8]
+    # the byte-lookup variant materializes ply 7 (166 GB at depth 8): it does
+    # not fit the 16 GiB arena, and the host-driven loop splits it instead
+    assert (st["splits"] == 0) == (v != 3)
+
+
+def test_graph_path_reference_positions(gdev, pg):
+    for name, item in golden("cpw.json").items():
+        for d, want in enumerate(item["perft"]):
+            for v in VARIANTS if d <= 5 else [0]:
+                assert gdev.perft(item["fen"], d, v) == want, (name, d, v)
+    for item in golden("c2.json")["positions"]:
+        for v in VARIANTS:
+            assert gdev.perft(pg.pos_from_words(item["words"]), 6, v) == item["perft"][6]
+    for item in golden("random.json")[:128]:
+        assert gdev.perft(pg.pos_from_words(item["words"]), 4, 0) == item["perft"][4]
+    for item in golden("edge.json"):
+        for d, want in enumerate(item["perft"]):
+            assert gdev.perft(pg.pos_from_words(item["words"]), d, 0) == want
+    for fen in TERMINAL:
+        for d in range(0, 4):
+            assert gdev.perft(fen, d, 0) == (1 if d == 0 else 0)
+
+
+def test_graph_path_overflow_is_reported(gdev, pg):
+    want = W.CPW_COUNTS["start"][6]
+    try:
+        gdev.set_count_bias((1 << 64) - want)
+        with pytest.raises(pg.PerftError) as e:
+            gdev.perft(W.CPW["start"], 6, 0)
+        assert e.value.code == pg.PG_EOVERFLOW
+        gdev.set_count_bias((1 << 64) - 1 - want)
+        assert gdev.perft(W.CPW["start"], 6, 0) == (1 << 64) - 1
+    finally:
+        gdev.set_count_bias(0)
