@@ -97,16 +97,18 @@ def leg_batch(pg, dev):
     arr = np.array(c3["words"], np.uint64)
     want = np.array(c3["perft"], np.uint64)
     total = int(want.sum())
-    space = {"slider": [0, 1, 2, 3], "dedup": [True, False], "blocks_per_cu": [4, 8], "chunk": [None]}
-    cfg, table = autotune.tune(dev, arr, 5, space=space, sample=512, log=log)
+    cfg, table = autotune.tune(dev, arr, 5, sample=512, log=log)
     runs = []
     for _ in range(3):
         t = time.perf_counter()
-        got, st = dev.perft_batch(arr, 5, cfg["slider"], cfg["dedup"], cfg["blocks_per_cu"])
-        runs.append((time.perf_counter() - t, st))
+        got = autotune.run(dev, arr, 5, cfg)
+        runs.append(time.perf_counter() - t)
         if not np.array_equal(got, want):
             raise SystemExit("MISMATCH in the 4096-position batch")
-    dt, st = min(runs, key=lambda r: r[0])
+    dt = min(runs)
+    got, st = dev.perft_batch(arr, 5, cfg["slider"], cfg["dedup"], cfg["blocks_per_cu"])   # stats of one launch
+    if not np.array_equal(got, want):
+        raise SystemExit("MISMATCH in the 4096-position batch")
     t = time.perf_counter()
     s = 0
     for w in arr:
@@ -117,9 +119,12 @@ def leg_batch(pg, dev):
     return {"workload": "4096 random positions (ply 5-80) at depth 5, one graph launch", "leaves": total,
             "ms": dt * 1e3, "graph_ms": st["graph_ms"], "leaves_per_s": total / dt, "launches": st["launches"],
             "autotuned": {"slider": pg.SLIDER_NAMES[cfg["slider"]], "dedup": cfg["dedup"],
-                          "blocks_per_cu": cfg["blocks_per_cu"]},
+                          "blocks_per_cu": cfg["blocks_per_cu"], "split_depth": cfg["split"],
+                          "grid": "sized to the batch" if cfg["grid"] < 0 else "whole chip"},
             "autotune_table": [{"slider": pg.SLIDER_NAMES[c["slider"]], "dedup": c["dedup"],
-                                "blocks_per_cu": c["blocks_per_cu"], "ms": t_ * 1e3} for c, t_ in table],
+                                "blocks_per_cu": c["blocks_per_cu"], "split_depth": c["split"],
+                                "grid": "batch" if c["grid"] < 0 else "chip", "threads_per_position": tpp,
+                                "ms_on_512_sample": t_ * 1e3} for c, t_, tpp in table],
             "one_position_per_launch": {"ms": loop * 1e3, "leaves_per_s": total / loop},
             "speedup_vs_plain_loop": loop / dt, "exact": True}
 
@@ -286,14 +291,16 @@ def leg_counters(pg, dev=None):
 
 
 def leg_serve(pg, dev):
-    """Served queries: Poisson streams at five rates and one burst, each over
-    its own slice of a pool of distinct seeded queries (no query repeats, so
-    TT hits are genuine transpositions).  The first 512 queries are the
-    golden set; every served result equals the plain one-query-per-launch
-    GPU path, which is also the throughput baseline."""
+    """Served queries through the library's serving loop (pg_serve_*): Poisson
+    streams at five rates and one burst, each over its own slice of a pool of
+    distinct seeded queries (no query repeats, so table hits are genuine
+    transpositions), with a shared 8 GB table; then the same burst without a
+    table, so batching alone is compared with the plain path.  The first 512
+    queries are the golden set; every served answer equals the plain
+    one-query-per-launch GPU path, which is also the throughput baseline."""
     import numpy as np
     from perft_gpu_amd import workloads as W
-    from perft_gpu_amd.serve import Server, serve, poisson_arrivals, latency_summary
+    from perft_gpu_amd.serve import ServeLoop, serve, poisson_arrivals, latency_summary
     c5 = golden("c5.json")
     plan = [(100, 200), (300, 600), (1000, 2000), (3000, 2000), (10000, 2000), (None, 2000)]
     pool, depths = W.c5_queries(sum(n for _, n in plan))
@@ -309,33 +316,41 @@ def leg_serve(pg, dev):
     if [int(x) for x in want[:len(c5["perft"])]] != c5["perft"]:
         raise SystemExit("MISMATCH in plain served-query loop")
     log("plain one-query-per-launch: %d queries %.2f s" % (len(pool), plain))
+
+    def row(loop, rate, idx):
+        arr = np.zeros(len(idx)) if rate is None else poisson_arrivals(len(idx), float(rate), seed=rate)
+        r = serve(loop, pool[idx], depths[idx], arr)
+        if not np.array_equal(r["results"], want[idx]):
+            raise SystemExit("MISMATCH in served results at rate %s" % rate)
+        leaves = int(want[idx].sum())
+        out = {"rate_per_s": rate if rate else "burst", "queries": len(idx), "elapsed_s": r["elapsed"],
+               "leaves_per_s": leaves / r["elapsed"], "batches": r["batches"], "mean_batch": r["mean_batch"],
+               "max_busy_slots": r["max_busy_slots"], "max_long_slots": r["max_long_slots"],
+               "plain_leaves_per_s": leaves / float(took[idx].sum())}
+        out.update(latency_summary(r["latency"]))
+        lat, dq = np.asarray(r["latency"]), depths[idx]
+        out["by_depth"] = {int(d): dict(latency_summary(lat[dq == d]), queries=int((dq == d).sum()))
+                           for d in sorted(set(int(x) for x in dq))}
+        log("serve rate %s: p50 %.2f ms p99 %.2f ms %.1f G leaves/s" % (rate, out["p50_ms"], out["p99_ms"],
+                                                                      out["leaves_per_s"] / 1e9))
+        return out
+
     rates, first = [], 0
-    with pg.TranspositionTable(dev, 8 << 30) as tt, Server(dev, slots=4, slot_bytes=16 << 30, tt=tt) as srv:
+    with pg.TranspositionTable(dev, 8 << 30) as tt, ServeLoop(dev, tt, slots=4, slot_bytes=16 << 30) as loop:
         for rate, n in plan:
-            idx = np.arange(first, first + n)
+            rates.append(row(loop, rate, np.arange(first, first + n)))
             first += n
-            arr = np.zeros(n) if rate is None else poisson_arrivals(n, float(rate), seed=rate)
-            r = serve(srv, pool[idx], depths[idx], arr)
-            if not np.array_equal(r["results"], want[idx]):
-                raise SystemExit("MISMATCH in served results at rate %s" % rate)
-            leaves = int(want[idx].sum())
-            row = {"rate_per_s": rate if rate else "burst", "queries": n, "elapsed_s": r["elapsed"],
-                   "leaves_per_s": leaves / r["elapsed"], "batches": r["batches"],
-                   "mean_batch": float(np.mean(r["batch_sizes"])), "max_busy_slots": r["max_busy_slots"]}
-            row["plain_leaves_per_s"] = leaves / float(took[idx].sum())
-            row.update(latency_summary(r["latency"]))
-            lat, dq = np.asarray(r["latency"]), depths[idx]
-            row["by_depth"] = {int(d): dict(latency_summary(lat[dq == d]), queries=int((dq == d).sum()))
-                               for d in sorted(set(int(x) for x in dq))}
-            rates.append(row)
-            log("serve rate %s: p50 %.2f ms p99 %.2f ms %.1f G leaves/s" % (rate, row["p50_ms"], row["p99_ms"],
-                                                                          row["leaves_per_s"] / 1e9))
+    burst_idx = np.arange(first - plan[-1][1], first)
+    with ServeLoop(dev, None, slots=4, slot_bytes=16 << 30) as loop:
+        no_tt = row(loop, None, burst_idx)
     burst = rates[-1]
     return {"workload": "%d distinct seeded queries (ply 0-60; depth 3/4/5/6/7 at 40/30/20/8/2%%), one slice per "
-                        "rate; Poisson arrivals at 100-10,000/s, then a burst of 2000 at once; 4 slots, shared 8 GB TT"
-                        % len(pool), "rates": rates,
+                        "rate; Poisson arrivals at 100-10,000/s, then a burst of 2000 at once; 4 slots, shared 8 GB TT; "
+                        "the burst again without a table" % len(pool), "rates": rates,
+            "burst_without_table": no_tt,
             "plain_one_query_per_launch": {"queries": len(pool), "s": plain, "leaves_per_s": int(want.sum()) / plain},
             "burst_vs_plain": burst["leaves_per_s"] / burst["plain_leaves_per_s"],
+            "burst_without_table_vs_plain": no_tt["leaves_per_s"] / no_tt["plain_leaves_per_s"],
             "exact": True}
 
 
@@ -394,6 +409,10 @@ def main():
               "(--gpus %d requested); multi-GPU sharding is a design note only." % args.gpus)
         sys.exit(2)
     legs_wanted = [x for x in args.legs.split(",") if x and x != "none"]
+    # the serving leg runs four slot streams next to the context's and the
+    # runtime's null stream: give each its own hardware queue (the HIP
+    # runtime reads this once, at its initialization, which is still ahead)
+    os.environ.setdefault("GPU_MAX_HW_QUEUES", "8")
 
     import perft_gpu_amd as pg
     # the counter passes run in child processes before this one touches the GPU

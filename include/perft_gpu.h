@@ -91,6 +91,10 @@ int pg_host_generate_moves(const pg_pos48 *pos, pg_move *out, uint32_t *n);
 /* MoveGeneratorBitboard::countMoves (:3010) of n boards on the host, with
  * one slider variant (the same source the kernels compile). */
 int pg_host_count_moves(const pg_pos48 *pos, uint32_t n, int slider, uint32_t *out);
+/* MoveGeneratorBitboard::generateBoards (:1302-1738) of n boards on the host:
+ * the children of board i follow those of board i-1 in out (at most cap of
+ * them in all; PG_ENOMEM beyond), counts[i] = their number. */
+int pg_host_children(const pg_pos48 *pos, uint32_t n, pg_pos48 *out, uint64_t cap, uint32_t *counts);
 /* MoveGeneratorBitboard::makeMove<chance, updateHash> (:3575); key may be NULL. */
 int pg_host_make_move(pg_pos48 *pos, pg_move move, pg_key128 *key);
 /* Slider lookups of one variant on the host tables (the device copies are
@@ -223,6 +227,7 @@ typedef struct pg_batch_stats {
     uint64_t splits;        /* halvings forced by the region budget */
     uint64_t leaf_items;    /* (parent, move) items of the leaf level */
     uint64_t graphs_cached; /* instantiated graphs held by the context */
+    uint64_t peak_bytes;    /* most bytes of the region the search used at once */
 } pg_batch_stats;
 
 /* Batched perft (no reference counterpart beyond launcher.h:966-1179's
@@ -234,6 +239,13 @@ typedef struct pg_batch_stats {
  * pos[i]. */
 int pg_perft_batch(pg_ctx *ctx, const pg_pos48 *pos, uint32_t n, uint32_t depth, int slider, int dedup,
                    int blocks_per_cu, pg_tt *tt, uint64_t *out, pg_batch_stats *stats);
+/* Work-groups per kernel of later pg_perft_batch calls (the threads that
+ * share a batch's positions): -1 (default) sized to the batch -- its largest
+ * level holds at most about n * 48^(depth-2) boards, 256 per work-group,
+ * rounded up to a power of two --, 0 the whole chip (blocks_per_cu per CU),
+ * > 0 at most that many.  Every kernel is a grid-stride loop, so the counts
+ * never depend on it. */
+int pg_set_batch_grid(pg_ctx *ctx, int max_groups);
 
 /* ------------------------------------------------------------- serving */
 
@@ -250,9 +262,66 @@ void pg_server_destroy(pg_server *srv);
 int pg_server_submit(pg_server *srv, uint32_t slot, const pg_pos48 *pos, uint32_t n, uint32_t depth, int slider,
                      int dedup);
 int pg_server_poll(pg_server *srv, uint32_t slot, uint64_t *out, pg_batch_stats *stats);
+/* pg_server_submit whose kernels keep at most wg_per_cu work-groups per CU
+ * resident (four fit per CU; 0 = no limit), leaving CUs to the other slots. */
+int pg_server_submit_capped(pg_server *srv, uint32_t slot, const pg_pos48 *pos, uint32_t n, uint32_t depth,
+                            int slider, int dedup, uint32_t wg_per_cu);
 int pg_server_slots(pg_server *srv, uint32_t *slots, uint64_t *bytes_per_slot);
 /* hipMemGetInfo of the context's device (admission by free HBM). */
 int pg_device_memory(pg_ctx *ctx, uint64_t *free_bytes, uint64_t *total_bytes);
+
+/* The serving loop (no reference counterpart; the reference runs one perft
+ * at a time, perftLauncher launcher.h:1455).  Queries (position, depth)
+ * arrive with pg_serve_submit; pg_serve_poll, called by the caller's thread,
+ * finishes completed batches, admits waiting queries into idle slots and
+ * returns finished queries with their latency (submit to completion seen).
+ * Admission: waiting queries are grouped by depth; the depth whose oldest
+ * query arrived first goes next, as many of its queries as max_batch and the
+ * slot's HBM region allow (region bytes per query of each depth are learned
+ * from the peak use of that depth's finished batches); depths >= long_depth
+ * hold at most long_slots slots, and their kernels at most long_wg_per_cu
+ * work-groups per CU, so short queries keep finding free slots and CUs.
+ * Zero fields take the defaults in brackets. */
+typedef struct pg_serve_config {
+    uint32_t slots;           /* concurrent batches, one stream each [GPU_MAX_HW_QUEUES - 2, 1..4]; each
+                                 slot's stream needs a hardware queue of its own (the context's and the
+                                 runtime's null stream take two), or its batches wait behind another's */
+    uint64_t bytes_per_slot;  /* HBM region per slot [60% of free HBM / slots, at most 24 GiB] */
+    uint32_t max_batch;       /* most queries per batch [1024] */
+    uint32_t long_depth;      /* queries of this depth or deeper are long [6] */
+    uint32_t long_slots;      /* most slots long batches hold at once [slots / 2] */
+    uint32_t long_wg_per_cu;  /* work-groups per CU a long batch's kernels keep resident [2] */
+    int slider;               /* slider variant [0 = Kogge-Stone; PG_SLIDER_AUTO = the context default] */
+    int dedup;                /* collapse identical positions within a level */
+} pg_serve_config;
+typedef struct pg_serve_result {
+    uint64_t id;        /* from pg_serve_submit */
+    uint64_t tag;       /* the caller's tag */
+    uint64_t perft;     /* the perft count (when status is PG_OK) */
+    uint32_t depth;
+    int status;         /* PG_OK or the error of the query's batch */
+    double latency_ms;  /* submit to completion seen by pg_serve_poll */
+} pg_serve_result;
+typedef struct pg_serve_stats {
+    uint64_t submitted, completed, batches, launches, splits, leaves;
+    uint32_t waiting, busy_slots, max_busy_slots, max_long_slots;
+    double p50_ms, p99_ms, mean_ms, max_ms;  /* latency of the completed queries */
+    double elapsed_ms;                       /* first submit to last completion */
+    double est_bytes_per_query[16];          /* current admission estimate per depth */
+} pg_serve_stats;
+typedef struct pg_serve pg_serve;
+int pg_serve_create(pg_ctx *ctx, pg_tt *tt /* may be NULL */, const pg_serve_config *cfg, pg_serve **out);
+void pg_serve_destroy(pg_serve *srv);
+int pg_serve_submit(pg_serve *srv, const pg_pos48 *pos, uint32_t depth, uint64_t tag, uint64_t *id);
+int pg_serve_poll(pg_serve *srv, pg_serve_result *out, uint32_t max, uint32_t *n);
+int pg_serve_get_stats(pg_serve *srv, pg_serve_stats *out);
+/* The admission decision alone (host only, no device): given waiting counts
+ * and oldest arrival time per depth, long batches running, idle slots, the
+ * region size and the per-depth estimates, returns 1 with the depth and the
+ * number of queries of the next batch, or 0 if none may start. */
+int pg_serve_policy(const uint32_t waiting[16], const double oldest[16], uint32_t busy_long, uint32_t idle_slots,
+                    const pg_serve_config *cfg, uint64_t region_bytes, const double est_bytes[16], uint32_t *depth,
+                    uint32_t *count);
 
 /* ---------------------------------------------------------- diagnostics */
 /* Measurement reference points (no reference counterpart; the reference's

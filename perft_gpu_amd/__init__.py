@@ -58,7 +58,7 @@ class Config(ctypes.Structure):
 
 class BatchStats(ctypes.Structure):
     _fields_ = [("total_ms", ctypes.c_double), ("graph_ms", ctypes.c_double), ("launches", u64), ("splits", u64),
-                ("leaf_items", u64), ("graphs_cached", u64)]
+                ("leaf_items", u64), ("graphs_cached", u64), ("peak_bytes", u64)]
 
     def as_dict(self):
         return {n: getattr(self, n) for n, _ in self._fields_}
@@ -114,6 +114,7 @@ def lib():
         "pg_zobrist128": (ctypes.c_int, [P(Pos48), P(Key128)]),
         "pg_host_generate_moves": (ctypes.c_int, [P(Pos48), P(ctypes.c_uint16), P(ctypes.c_uint32)]),
         "pg_host_make_move": (ctypes.c_int, [P(Pos48), ctypes.c_uint16, P(Key128)]),
+        "pg_host_children": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_uint32, ctypes.c_void_p, u64, ctypes.c_void_p]),
         "pg_host_count_moves": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_uint32, ctypes.c_int, ctypes.c_void_p]),
         "pg_host_slider_attacks": (ctypes.c_int, [ctypes.c_int, ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p,
                                                   ctypes.c_uint32, ctypes.c_void_p]),
@@ -150,6 +151,7 @@ def lib():
         "pg_perft_tt": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, P(Pos48), ctypes.c_uint32, ctypes.c_int,
                                        P(u64)]),
         "pg_set_dedup": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int]),
+        "pg_set_batch_grid": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int]),
         "pg_perft_deep": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, P(Pos48), ctypes.c_uint32, ctypes.c_uint32,
                                          ctypes.c_uint32, ctypes.c_int, DIVIDE_CB, ctypes.c_void_p, P(u64),
                                          P(DeepStats)]),
@@ -215,6 +217,21 @@ def count_moves_host(positions, slider):
     out = np.zeros(len(arr), dtype=np.uint32)
     _check(lib().pg_host_count_moves(_ptr(arr), len(arr), slider, _ptr(out)), "pg_host_count_moves")
     return out
+
+
+def children_host(positions, plies=1):
+    """generateBoards on the host, `plies` times: (children (m,6) array, owner
+    index of each child in the input)."""
+    arr = positions_array(positions)
+    owner = np.arange(len(arr), dtype=np.int64)
+    for _ in range(plies):
+        cap = max(1, len(arr) * 218)
+        out = np.zeros((cap, 6), dtype=np.uint64)
+        cnt = np.zeros(len(arr), dtype=np.uint32)
+        _check(lib().pg_host_children(_ptr(arr), len(arr), _ptr(out), cap, _ptr(cnt)), "pg_host_children")
+        owner = np.repeat(owner, cnt.astype(np.int64))
+        arr = np.ascontiguousarray(out[:int(cnt.sum())])
+    return arr, owner
 
 
 def make_move_host(pos, move, key=None):
@@ -301,6 +318,10 @@ class Device:
     def set_count_bias(self, bias):
         """Test hook pg_test_count_bias: later perft totals start at `bias`."""
         _check(lib().pg_test_count_bias(self._h, bias), "pg_test_count_bias")
+
+    def set_batch_grid(self, max_groups):
+        """pg_set_batch_grid: -1 sized to each batch, 0 whole chip, > 0 at most that many work-groups."""
+        _check(lib().pg_set_batch_grid(self._h, max_groups), "pg_set_batch_grid")
 
     def set_slider(self, slider):
         _check(lib().pg_set_slider(self._h, slider), "pg_set_slider")

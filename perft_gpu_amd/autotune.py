@@ -1,38 +1,84 @@
 """Autotuner for batched perft.
 
-Picks, for a batch workload (positions, depth), the slider variant, whether
-to collapse duplicates within levels, the persistent grid size (work-groups
-per CU, i.e. how many threads share the batch's positions) and how many
-positions go into one graph launch.  Candidates are timed on a seeded sample
-of the batch; every timed run is also checked against the first candidate's
-counts, so a configuration that changed a result could never win.
+For a batch workload (positions, depth) it times, on a seeded sample of the
+positions, every combination of
+  slider         slider-attack variant (Kogge-Stone, plain / fixed-shift /
+                 byte-lookup magics)
+  dedup          collapse identical positions within a level
+  blocks_per_cu  work-groups per CU of the persistent grid-stride kernels
+  split          split depth: the positions are first expanded `split` plies
+                 on the host (generateBoards) and perft(depth - split) of all
+                 the resulting positions runs as one batch, folded back per
+                 root -- the batched analogue of the reference's launch depth
+                 (estimateLaunchDepth, launcher.h:1535-1563; perft.cu:117-120)
+  grid           work-groups per kernel, i.e. how many threads share the
+                 batch's positions: -1 sized to the batch's largest level, 0
+                 the whole chip (pg_set_batch_grid)
+  chunk          positions per graph launch (None: the whole batch)
+and returns the fastest.  Every timed run is checked against the first
+configuration's counts, so a configuration that changed a result could never
+win.  The reference has no autotuner; its choices are compile-time switches
+(switches.h) and a fixed launch depth.
 """
 import time
 
 import numpy as np
 
-from . import SLIDER_NAMES
+from . import SLIDER_NAMES, children_host
 
 DEFAULT_SPACE = {
     "slider": [0, 1, 2, 3],
     "dedup": [True, False],
-    "blocks_per_cu": [4, 8, 16],
-    "chunk": [None],             # None = whole batch in one launch
+    "blocks_per_cu": [4, 8],
+    "split": [0, 1],
+    "grid": [-1, 0],
+    "chunk": [None],
 }
 
 
-def _run(dev, arr, depth, cfg):
-    chunk = cfg["chunk"] or len(arr)
-    out = []
-    for i in range(0, len(arr), chunk):
-        c, _ = dev.perft_batch(arr[i:i + chunk], depth, cfg["slider"], cfg["dedup"], cfg["blocks_per_cu"])
-        out.append(c)
-    return np.concatenate(out)
+def work_groups(dev_cus, n, depth, grid, blocks_per_cu):
+    """Work-groups of the batch's grid-stride kernels (mirrors pg_set_batch_grid)."""
+    full = dev_cus * 64
+    if grid > 0:
+        return grid
+    if grid == 0:
+        return dev_cus * blocks_per_cu
+    wgs = n / 256.0
+    for _ in range(2, depth):
+        if wgs >= full:
+            break
+        wgs *= 48.0
+    if wgs >= full:
+        return dev_cus * blocks_per_cu
+    g = 64
+    while g < wgs:
+        g <<= 1
+    return min(g, dev_cus * blocks_per_cu)
 
 
-def tune(dev, positions, depth, space=None, sample=512, seed=1, repeats=1, log=None):
-    """Returns (best_config, table of (config, seconds))."""
-    space = space or DEFAULT_SPACE
+def run(dev, arr, depth, cfg):
+    """perft(depth) of every row of arr under one configuration."""
+    split = min(cfg.get("split", 0), depth)
+    work, owner = (children_host(arr, split) if split else (arr, np.arange(len(arr))))
+    d = depth - split
+    chunk = cfg.get("chunk") or len(work)
+    dev.set_batch_grid(cfg.get("grid", -1))
+    try:
+        parts = []
+        for i in range(0, len(work), chunk):
+            c, _ = dev.perft_batch(work[i:i + chunk], d, cfg["slider"], cfg["dedup"], cfg["blocks_per_cu"])
+            parts.append(c)
+    finally:
+        dev.set_batch_grid(-1)
+    counts = np.concatenate(parts) if parts else np.zeros(0, np.uint64)
+    out = np.zeros(len(arr), dtype=np.uint64)
+    np.add.at(out, owner, counts)
+    return out
+
+
+def tune(dev, positions, depth, space=None, sample=512, seed=1, repeats=1, log=None, cus=256):
+    """Returns (best_config, table of (config, seconds, threads_per_position))."""
+    space = dict(DEFAULT_SPACE, **(space or {}))
     arr = np.ascontiguousarray(positions, dtype=np.uint64).reshape(-1, 6)
     rng = np.random.default_rng(seed)
     if len(arr) > sample:
@@ -42,20 +88,26 @@ def tune(dev, positions, depth, space=None, sample=512, seed=1, repeats=1, log=N
     for s in space["slider"]:
         for dd in space["dedup"]:
             for b in space["blocks_per_cu"]:
-                for ch in space["chunk"]:
-                    cfg = {"slider": s, "dedup": dd, "blocks_per_cu": b, "chunk": ch}
-                    _run(dev, arr, depth, cfg)      # graph capture + warm caches
-                    best = float("inf")
-                    for _ in range(repeats):
-                        t = time.perf_counter()
-                        got = _run(dev, arr, depth, cfg)
-                        best = min(best, time.perf_counter() - t)
-                    if ref is None:
-                        ref = got
-                    elif not np.array_equal(ref, got):
-                        raise RuntimeError("autotune: counts differ for %s" % cfg)
-                    table.append((cfg, best))
-                    if log:
-                        log("autotune %s dedup=%d bpc=%d chunk=%s: %.2f ms" % (SLIDER_NAMES[s], dd, b, ch, best * 1e3))
+                for sp in space["split"]:
+                    for g in space["grid"]:
+                        for ch in space["chunk"]:
+                            cfg = {"slider": s, "dedup": dd, "blocks_per_cu": b, "split": sp, "grid": g, "chunk": ch}
+                            run(dev, arr, depth, cfg)      # graph capture + warm caches
+                            best = float("inf")
+                            for _ in range(repeats):
+                                t = time.perf_counter()
+                                got = run(dev, arr, depth, cfg)
+                                best = min(best, time.perf_counter() - t)
+                            if ref is None:
+                                ref = got
+                            elif not np.array_equal(ref, got):
+                                raise RuntimeError("autotune: counts differ for %s" % cfg)
+                            n_work = len(arr) * (30 ** sp)
+                            tpp = work_groups(cus, n_work, depth - sp, g, b) * 256 / max(1, len(arr))
+                            table.append((cfg, best, tpp))
+                            if log:
+                                log("autotune %s dedup=%d bpc=%d split=%d grid=%s chunk=%s: %.2f ms"
+                                    % (SLIDER_NAMES[s], dd, b, sp, "batch" if g < 0 else ("chip" if g == 0 else g),
+                                       ch, best * 1e3))
     best_cfg = min(table, key=lambda x: x[1])[0]
     return best_cfg, table

@@ -181,6 +181,30 @@ int pg_host_count_moves(const pg_pos48 *pos, uint32_t n, int slider, uint32_t *o
     return PG_OK;
 }
 
+int pg_host_children(const pg_pos48 *pos, uint32_t n, pg_pos48 *out, uint64_t cap, uint32_t *counts) {
+    const HostTables *h = tables_or_null();
+    if (!h) return PG_ETABLE;
+    if (n && (!pos || !counts)) return PG_EINVAL;
+    const Tables t = h->view();
+    u64 w = 0;
+    u16 mv[256];
+    for (uint32_t i = 0; i < n; i++) {
+        Pos p;
+        memcpy(&p, &pos[i], sizeof(p));
+        if (!pos_sane(p)) return PG_EINVAL;
+        const u32 c = generate_any<SL_FANCY>(p, t, mv, 256);
+        counts[i] = c;
+        if (w + c > cap) return PG_ENOMEM;
+        for (u32 j = 0; j < c; j++) {
+            Pos q = p;
+            make_any(q, mv[j], t);
+            memcpy(&out[w + j], &q, sizeof(q));
+        }
+        w += c;
+    }
+    return PG_OK;
+}
+
 int pg_host_make_move(pg_pos48 *pos, pg_move move, pg_key128 *key) {
     const HostTables *h = tables_or_null();
     if (!h) return PG_ETABLE;

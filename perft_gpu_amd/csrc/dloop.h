@@ -41,6 +41,7 @@ struct Plan {
     u64 acc;
     u64 bias;           // test hook: starting value of every root's total
     u32 ovf;            // a 64-bit total wrapped
+    u64 peak;           // highest bump top reached (bytes of the region used)
     DLevel lv[TT_MAX_DEPTH];
 };
 
@@ -51,6 +52,7 @@ __device__ __forceinline__ void *plan_alloc(Plan *P, u64 bytes) {
         return nullptr;
     }
     P->top = a + bytes;
+    if (P->top > P->peak) P->peak = P->top;
     return P->base + a;
 }
 

@@ -48,6 +48,7 @@ struct Engine {
     u64 *dItems = nullptr;   // items consumed by fused leaf launches
     int leafMode = 0;        // 0 fused last two plies, 1 materialized item list
     bool graphPerft = true;  // pg_perft may run as one graph launch
+    int batchGrid = -1;      // work-groups per kernel of pg_perft_batch: -1 sized to the batch, 0 whole chip
     u32 *dErr = nullptr;
     u32 *dOvf = nullptr;     // set when a 64-bit total wraps
     u64 bias = 0;            // test hook: starting value of every root total
@@ -84,6 +85,9 @@ void slot_forget_tt(Slot &sl, const TTable *tt);  // drop graphs that bake this 
 int batch_perft(Engine &e, Slot &sl, TTable *tt, const Pos *roots, u64 n, u32 depth, int v, int dedup, int bpc,
                 u64 *out, pg_batch_stats *bs);
 int perft_graph(Engine &e, const Pos &root, u32 depth, int v, u64 *out, bool *fits);
+struct Server;  // the object behind pg_server
+int server_submit(Server &s, uint32_t slot, const pg_pos48 *pos, uint32_t n, uint32_t depth, int slider, int dedup,
+                  int wgCap);
 
 // Device transposition table (the object behind pg_tt).
 struct TTable {

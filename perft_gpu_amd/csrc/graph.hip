@@ -23,11 +23,52 @@ struct GraphShape {
     int depth, v, keys, dedup, bpc;
     const TTable *tt;
     int total;  // 1: one root, only the sum is wanted (no per-board perft arrays)
+    int cap;    // > 0: at most this many work-groups per CU in every kernel (leaves CUs to other streams)
+    int gmax;   // > 0: at most this many work-groups in any kernel (a small search's whole need)
     bool operator==(const GraphShape &o) const {
         return depth == o.depth && v == o.v && keys == o.keys && dedup == o.dedup && bpc == o.bpc && tt == o.tt &&
-               total == o.total;
+               total == o.total && cap == o.cap && gmax == o.gmax;
     }
 };
+
+// The most work-groups any kernel of a search of n roots to `depth` can use:
+// its largest materialized level holds at most about n * 48^(depth-2)
+// boards, 256 per work-group.  Rounded up to a power of two (at least 64) so
+// that few distinct graphs are captured; 0 when the whole chip is wanted.
+// Every kernel is a grid-stride loop, so any grid size is correct; a small
+// grid only spares the dispatcher the empty work-groups of a small search.
+static int grid_limit(const Engine &e, u64 n, int depth) {
+    const int full = e.cus * LEAF_BLOCKS_PER_CU;
+    double wgs = (double)n / 256.0;
+    for (int l = 2; l < depth && wgs < full; l++) wgs *= 48.0;
+    if (wgs >= full) return 0;
+    int g = 64;
+    while (g < wgs) g <<= 1;
+    return g >= full ? 0 : g;
+}
+
+// Grid sizes of one captured search
+struct Grids {
+    int work, scan, byteV, leaf, v;  // grid-stride kernels, scans, byte-lookup kernels, fused leaf, per-variant
+};
+static Grids grids_for(const Engine &e, const GraphShape &sh) {
+    // a capped search keeps at most sh.cap work-groups per CU of each kernel
+    // resident (four fit per CU), so batches on other streams find room
+    const int wpc = sh.cap > 0 && sh.cap < sh.bpc ? sh.cap : sh.bpc;
+    Grids g;
+    g.work = e.cus * wpc;
+    g.scan = e.cus * (sh.cap > 0 && sh.cap < 4 ? sh.cap : 4);
+    g.byteV = sh.cap > 0 && sh.cap < 4 ? (e.cus * sh.cap + 3) / 4 : e.cus;
+    g.leaf = e.cus * (sh.cap > 0 ? sh.cap : LEAF_BLOCKS_PER_CU);
+    if (sh.gmax > 0) {
+        g.work = std::min(g.work, sh.gmax);
+        g.scan = std::min(g.scan, sh.gmax);
+        g.byteV = std::min(g.byteV, sh.gmax);
+        g.leaf = std::min(g.leaf, sh.gmax);
+    }
+    g.v = sh.v == SL_BYTE ? g.byteV : g.work;
+    return g;
+}
 
 struct BatchGraph {
     GraphShape shape;
@@ -67,8 +108,8 @@ template <int V> static void cap_leaf_total(hipStream_t s, Plan *P, int l, int g
 // plan's total.
 static void enqueue_total(const Engine &e, hipStream_t s, Plan *P, const GraphShape &sh) {
     const int D = sh.depth, v = sh.v;
-    const int gWork = e.cus * sh.bpc, gScan = e.cus * 4;
-    const int gv = v == SL_BYTE ? e.cus : gWork;
+    const Grids G = grids_for(e, sh);
+    const int gWork = G.work, gScan = G.scan, gv = G.v;
     PG_VARIANT_CALL(v, cap_count, s, P, gv, e.dt);
     const int lastExpand = v == SL_BYTE ? D - 2 : D - 3;
     for (int l = 0; l <= lastExpand; l++) {
@@ -81,14 +122,14 @@ static void enqueue_total(const Engine &e, hipStream_t s, Plan *P, const GraphSh
     }
     const int last = lastExpand + 1 < 0 ? 0 : lastExpand + 1;  // deepest materialized level
     if (D == 1 || v == SL_BYTE) kd_sum_counts<<<gWork, 256, 0, s>>>(P, last);
-    else PG_VARIANT_CALL(v, cap_leaf_total, s, P, last, e.cus * LEAF_BLOCKS_PER_CU, e.dt);
+    else PG_VARIANT_CALL(v, cap_leaf_total, s, P, last, G.leaf, e.dt);
 }
 
 // Enqueues the whole search for `depth` on stream s (captured by the caller).
 static void enqueue_search(const Engine &e, hipStream_t s, Plan *P, const GraphShape &sh) {
     const int D = sh.depth, v = sh.v;
-    const int gWork = e.cus * sh.bpc, gByte = e.cus, gScan = e.cus * 4;
-    const int gv = v == SL_BYTE ? gByte : gWork;
+    const Grids G = grids_for(e, sh);
+    const int gWork = G.work, gScan = G.scan, gLeaf = G.leaf, gv = G.v;
     TTView tt{};
     TTCounters *ctr = nullptr;
     if (sh.tt) {
@@ -118,7 +159,7 @@ static void enqueue_search(const Engine &e, hipStream_t s, Plan *P, const GraphS
     if (D == 1 || v == SL_BYTE) {
         kd_perft1<<<gWork, 256, 0, s>>>(P, last < 0 ? 0 : last);
     } else {
-        PG_VARIANT_CALL(v, cap_leaf, s, P, last, e.cus * LEAF_BLOCKS_PER_CU, e.dt);
+        PG_VARIANT_CALL(v, cap_leaf, s, P, last, gLeaf, e.dt);
     }
     for (int l = lastExpand; l >= 0; l--) {
         kd_resolve<<<gWork, 256, 0, s>>>(P, l, tt, D - l - 1, ctr);
@@ -149,7 +190,7 @@ static int get_graph(Engine &e, Slot &sl, const GraphShape &sh, hipGraphExec_t *
 // Runs one batch (sub-range of the level-0 arrays already on the device);
 // sets *overflow if it did not fit.
 static int run_range(Engine &e, Slot &sl, const GraphShape &sh, u64 first, u64 n, const DLevel &L0, size_t top0,
-                     bool *overflow, u64 *leafItems, double *ms) {
+                     bool *overflow, u64 *leafItems, double *ms, u64 *peak) {
     Plan &h = *sl.hPlan;
     memset(&h, 0, sizeof(Plan));
     h.base = sl.region;
@@ -178,6 +219,7 @@ static int run_range(Engine &e, Slot &sl, const GraphShape &sh, u64 first, u64 n
     *ms += t;
     *overflow = h.overflow != 0;
     *leafItems += h.leafItems;
+    if (h.peak > *peak) *peak = h.peak;
     if (h.ovf) {
         set_error("batch: a perft total does not fit 64 bits");
         return PG_EOVERFLOW;
@@ -190,10 +232,10 @@ static int run_range(Engine &e, Slot &sl, const GraphShape &sh, u64 first, u64 n
 }
 
 static int run_split(Engine &e, Slot &sl, const GraphShape &sh, u64 first, u64 n, const DLevel &L0, size_t top0,
-                     u64 *leafItems, double *ms, u64 *launches, u64 *splits) {
+                     u64 *leafItems, double *ms, u64 *launches, u64 *splits, u64 *peak) {
     bool over = false;
     (*launches)++;
-    int rc = run_range(e, sl, sh, first, n, L0, top0, &over, leafItems, ms);
+    int rc = run_range(e, sl, sh, first, n, L0, top0, &over, leafItems, ms, peak);
     if (rc) return rc;
     if (!over) return PG_OK;
     if (n == 1) {
@@ -202,9 +244,9 @@ static int run_split(Engine &e, Slot &sl, const GraphShape &sh, u64 first, u64 n
     }
     (*splits)++;
     const u64 h = n / 2;
-    rc = run_split(e, sl, sh, first, h, L0, top0, leafItems, ms, launches, splits);
+    rc = run_split(e, sl, sh, first, h, L0, top0, leafItems, ms, launches, splits, peak);
     if (rc) return rc;
-    return run_split(e, sl, sh, first + h, n - h, L0, top0, leafItems, ms, launches, splits);
+    return run_split(e, sl, sh, first + h, n - h, L0, top0, leafItems, ms, launches, splits, peak);
 }
 
 int slot_init(Engine &e, Slot &sl, u8 *region, size_t bytes, hipStream_t stream) {
@@ -268,7 +310,7 @@ int perft_graph(Engine &e, const Pos &root, u32 depth, int v, u64 *out, bool *fi
     h.lv[0].cnt = (u32 *)(sl.region + 256);
     h.top = 512;
     h.acc = e.bias;
-    GraphShape sh{(int)depth, v, 0, 0, e.bpc, nullptr, 1};
+    GraphShape sh{(int)depth, v, 0, 0, e.bpc, nullptr, 1, 0, grid_limit(e, 1, (int)depth)};
     hipGraphExec_t exec;
     int rc = get_graph(e, sl, sh, &exec);
     if (rc) return rc;
@@ -344,10 +386,11 @@ int batch_perft(Engine &e, Slot &sl, TTable *tt, const Pos *roots, u64 n, u32 de
     }
     HIP_TRY(hipMemcpyAsync(L0.b, hb.data(), n * sizeof(Pos), hipMemcpyHostToDevice, sl.stream));
     if (keys) HIP_TRY(hipMemcpyAsync(L0.k, hk.data(), n * sizeof(Key128), hipMemcpyHostToDevice, sl.stream));
-    GraphShape sh{(int)depth, v, keys ? 1 : 0, dedup ? 1 : 0, bpc > 0 ? bpc : e.bpc, tt, 0};
-    u64 leafItems = 0, launches = 0, splits = 0;
+    GraphShape sh{(int)depth, v, keys ? 1 : 0, dedup ? 1 : 0, bpc > 0 ? bpc : e.bpc, tt, 0, 0,
+                  e.batchGrid >= 0 ? e.batchGrid : grid_limit(e, n, (int)depth)};
+    u64 leafItems = 0, launches = 0, splits = 0, peak = 0;
     double ms = 0;
-    int rc = run_split(e, sl, sh, 0, n, L0, top, &leafItems, &ms, &launches, &splits);
+    int rc = run_split(e, sl, sh, 0, n, L0, top, &leafItems, &ms, &launches, &splits, &peak);
     if (rc) return rc;
     std::vector<u64> res(n);
     HIP_TRY(hipMemcpyAsync(res.data(), L0.perft, n * 8, hipMemcpyDeviceToHost, sl.stream));
@@ -360,6 +403,7 @@ int batch_perft(Engine &e, Slot &sl, TTable *tt, const Pos *roots, u64 n, u32 de
         bs->leaf_items = leafItems;
         bs->total_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
         bs->graphs_cached = sl.graphs.size();
+        bs->peak_bytes = peak;
     }
     return PG_OK;
 }
@@ -368,6 +412,12 @@ int batch_perft(Engine &e, Slot &sl, TTable *tt, const Pos *roots, u64 n, u32 de
 
 // ====================================================================== C ABI
 extern "C" {
+
+int pg_set_batch_grid(pg_ctx *ctx, int max_groups) {
+    if (!ctx || max_groups < -1) return PG_EINVAL;
+    reinterpret_cast<Engine *>(ctx)->batchGrid = max_groups;
+    return PG_OK;
+}
 
 int pg_perft_batch(pg_ctx *ctx, const pg_pos48 *pos, uint32_t n, uint32_t depth, int slider, int dedup,
                    int blocks_per_cu, pg_tt *tt, uint64_t *out, pg_batch_stats *st) {
@@ -411,7 +461,7 @@ struct ServerSlot {
     bool busy = false;
     u64 n = 0, first = 0;
     u32 depth = 0;
-    int v = 0, dedup = 0;
+    int v = 0, dedup = 0, wgCap = 0;
     std::vector<u64> order;
     u64 *hOut = nullptr;  // pinned results, slot capacity
     u64 cap = 0;
@@ -499,7 +549,24 @@ void pg_server_destroy(pg_server *s) {
 // Stage a batch on an idle slot and launch its graph; returns at once.
 int pg_server_submit(pg_server *s, uint32_t slot, const pg_pos48 *pos, uint32_t n, uint32_t depth, int slider,
                      int dedup) {
-    if (!s || slot >= s->slots.size() || !pos || n == 0 || depth < 1 || depth >= TT_MAX_DEPTH) return PG_EINVAL;
+    if (!s) return PG_EINVAL;
+    return pg::server_submit(*s, slot, pos, n, depth, slider, dedup, 0);
+}
+
+int pg_server_submit_capped(pg_server *s, uint32_t slot, const pg_pos48 *pos, uint32_t n, uint32_t depth, int slider,
+                            int dedup, uint32_t wg_per_cu) {
+    if (!s || wg_per_cu > 64) return PG_EINVAL;
+    return pg::server_submit(*s, slot, pos, n, depth, slider, dedup, (int)wg_per_cu);
+}
+
+}  // extern "C"
+
+// wgCap > 0: the batch's kernels keep at most wgCap work-groups per CU
+// resident, so that batches on the other slots find room on every CU.
+int pg::server_submit(Server &srv, uint32_t slot, const pg_pos48 *pos, uint32_t n, uint32_t depth, int slider,
+                      int dedup, int wgCap) {
+    pg_server *s = static_cast<pg_server *>(&srv);
+    if (slot >= s->slots.size() || !pos || n == 0 || depth < 1 || depth >= TT_MAX_DEPTH || wgCap < 0) return PG_EINVAL;
     ServerSlot &ss = s->slots[slot];
     if (ss.busy) return PG_EINVAL;
     if (n > ss.cap) return PG_EINVAL;
@@ -541,7 +608,7 @@ int pg_server_submit(pg_server *s, uint32_t slot, const pg_pos48 *pos, uint32_t 
     if (!L0.dup) return PG_ENOMEM;
     HIP_TRY(hipMemcpyAsync(L0.b, hb.data(), n * sizeof(Pos), hipMemcpyHostToDevice, sl.stream));
     HIP_TRY(hipMemcpyAsync(L0.k, hk.data(), n * sizeof(Key128), hipMemcpyHostToDevice, sl.stream));
-    GraphShape sh{(int)depth, v, 1, dedup ? 1 : 0, e.bpc, s->tt, 0};
+    GraphShape sh{(int)depth, v, 1, dedup ? 1 : 0, e.bpc, s->tt, 0, wgCap, grid_limit(e, n, (int)depth)};
     hipGraphExec_t exec;
     int rc = get_graph(e, sl, sh, &exec);
     if (rc) return rc;
@@ -562,11 +629,14 @@ int pg_server_submit(pg_server *s, uint32_t slot, const pg_pos48 *pos, uint32_t 
     ss.depth = depth;
     ss.v = v;
     ss.dedup = dedup;
+    ss.wgCap = wgCap;
     ss.L0 = L0;
     ss.top0 = top;
     ss.t0 = host_ms();
     return PG_OK;
 }
+
+extern "C" {
 
 // 1: done (out[] filled, in submission order), 0: still running, < 0: error.
 // A batch that overflowed its region is finished here synchronously in
@@ -590,14 +660,15 @@ int pg_server_poll(pg_server *s, uint32_t slot, uint64_t *out, pg_batch_stats *s
         set_error("device consistency check failed in served batch (flag %u)", h.err);
         return PG_EINVAL;
     }
-    u64 items = h.leafItems, launches = 1, splits = 0;
+    u64 items = h.leafItems, launches = 1, splits = 0, peak = h.peak;
     double ms = 0;
     if (h.overflow) {
-        GraphShape sh{(int)ss.depth, ss.v, 1, ss.dedup ? 1 : 0, e.bpc, s->tt, 0};
+        GraphShape sh{(int)ss.depth, ss.v, 1, ss.dedup ? 1 : 0, e.bpc, s->tt, 0, ss.wgCap,
+                      grid_limit(e, ss.n, (int)ss.depth)};
         (void)hipSetDevice(e.device);
         items = 0;
         launches = 0;
-        int rc = run_split(e, sl, sh, 0, ss.n, ss.L0, ss.top0, &items, &ms, &launches, &splits);
+        int rc = run_split(e, sl, sh, 0, ss.n, ss.L0, ss.top0, &items, &ms, &launches, &splits, &peak);
         if (rc) return rc;
         HIP_TRY(hipMemcpyAsync(ss.hOut, ss.L0.perft, ss.n * 8, hipMemcpyDeviceToHost, sl.stream));
         HIP_TRY(hipStreamSynchronize(sl.stream));
@@ -611,6 +682,7 @@ int pg_server_poll(pg_server *s, uint32_t slot, uint64_t *out, pg_batch_stats *s
         st->splits = splits;
         st->leaf_items = items;
         st->graphs_cached = sl.graphs.size();
+        st->peak_bytes = peak;
     }
     return 1;
 }

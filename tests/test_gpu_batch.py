@@ -85,3 +85,35 @@ def test_c3_batch_4096_positions_depth5(dev):
     got, st = dev.perft_batch(arr, 5, 0, True)
     assert [int(x) for x in got] == c3["perft"]
     assert st["launches"] == 1
+
+
+def test_batch_grid_sizes_never_change_counts(dev):
+    arr, items = random_set(256)
+    want = [it["perft"][4] for it in items]
+    try:
+        for g in (1, 64, 1024, 0, -1):
+            print("batch grid", g, flush=True)
+            dev.set_batch_grid(g)
+            got, _ = dev.perft_batch(arr, 4, 0, True)
+            assert [int(x) for x in got] == want
+    finally:
+        dev.set_batch_grid(-1)
+
+
+def test_split_depth_and_autotuner(dev, pg):
+    """A batch expanded one or two plies on the host first (split depth)
+    folds back to the same per-root counts; the autotuner's choice, run on
+    the whole set, reproduces the reference's values."""
+    from perft_gpu_amd import autotune
+    arr, items = random_set(128)
+    want = np.array([it["perft"][4] for it in items], np.uint64)
+    for split in (0, 1, 2):
+        for grid in (-1, 0):
+            cfg = {"slider": 0, "dedup": True, "blocks_per_cu": 8, "split": split, "grid": grid, "chunk": None}
+            assert np.array_equal(autotune.run(dev, arr, 4, cfg), want), cfg
+    cfg = {"slider": 2, "dedup": False, "blocks_per_cu": 4, "split": 1, "grid": 0, "chunk": 32}
+    assert np.array_equal(autotune.run(dev, arr, 4, cfg), want)
+    space = {"slider": [0, 2], "dedup": [True], "blocks_per_cu": [8], "split": [0, 1], "grid": [-1, 0]}
+    best, table = autotune.tune(dev, arr, 4, space=space, sample=64)
+    assert len(table) == 8 and best in [c for c, _, _ in table]
+    assert np.array_equal(autotune.run(dev, arr, 4, best), want)

@@ -15,28 +15,69 @@ def c5():
 
 
 def test_serve_mixed_stream_exact(dev, pg):
-    from perft_gpu_amd.serve import Server, serve, poisson_arrivals
+    """256 queries arriving at 2000/s through the library's serving loop with a
+    shared table: every answer equals the reference's value and the same query
+    alone; queries share batches, batches overlap on several slots, long
+    batches keep to their share of the slots."""
+    from perft_gpu_amd.serve import ServeLoop, serve, poisson_arrivals
     pos, depths, want = c5()
     n = 256
-    with pg.TranspositionTable(dev, 2 << 30) as tt, Server(dev, slots=4, slot_bytes=4 << 30, tt=tt) as srv:
+    with pg.TranspositionTable(dev, 2 << 30) as tt, ServeLoop(dev, tt, slots=4, slot_bytes=4 << 30) as loop:
         arr = poisson_arrivals(n, 2000.0, seed=7)
         print("serve", n, "queries", flush=True)
-        out = serve(srv, pos[:n], depths[:n], arr)
+        out = serve(loop, pos[:n], depths[:n], arr)
+        st = loop.stats()
     assert [int(x) for x in out["results"]] == want[:n]
-    assert max(out["batch_sizes"]) > 1          # queries shared a batch
-    assert out["max_busy_slots"] >= 2           # batches overlapped on the device
+    assert st["completed"] == n and st["batches"] < n   # queries shared a batch
+    assert st["max_busy_slots"] >= 2                     # batches overlapped on the device
+    assert st["max_long_slots"] <= 2
+    assert 0 < st["p50_ms"] <= st["p99_ms"] <= st["max_ms"]
     # the same query alone, on the plain path
     for i in range(0, n, 37):
         assert dev.perft(pg.pos_from_words(pos[i]), int(depths[i]), 0) == want[i]
 
 
-def test_serve_burst_all_at_once(dev, pg):
-    from perft_gpu_amd.serve import Server, serve
+def test_serve_burst_all_at_once_without_table(dev, pg):
+    from perft_gpu_amd.serve import ServeLoop, serve
     pos, depths, want = c5()
     n = 128
-    with Server(dev, slots=2, slot_bytes=4 << 30, tt=None) as srv:
-        out = serve(srv, pos[:n], depths[:n], np.zeros(n))
+    with ServeLoop(dev, None, slots=2, slot_bytes=4 << 30) as loop:
+        out = serve(loop, pos[:n], depths[:n], np.zeros(n))
     assert [int(x) for x in out["results"]] == want[:n]
+    assert out["mean_batch"] > 1
+
+
+def test_serve_learns_region_bytes_and_splits_small_regions(dev, pg):
+    """With 64 MiB regions the loop sizes depth-5 batches by its region-bytes
+    estimate, learns the estimate from the finished batches, and every answer
+    stays exact."""
+    from perft_gpu_amd.serve import ServeLoop, serve
+    g = golden("c2.json")["positions"]
+    arr = np.array([x["words"] for x in g] * 4, np.uint64)
+    want = [x["perft"][5] for x in g] * 4
+    with ServeLoop(dev, None, slots=1, slot_bytes=64 << 20, max_batch=64) as loop:
+        out = serve(loop, arr, np.full(len(arr), 5, np.uint32), np.zeros(len(arr)))
+        st = loop.stats()
+    assert [int(x) for x in out["results"]] == want
+    assert st["batches"] >= 2 and st["est_bytes_per_query"][5] > 0
+
+
+def test_serve_rejects_bad_queries(dev, pg):
+    from perft_gpu_amd.serve import ServeLoop
+    g = golden("c2.json")["positions"]
+    with ServeLoop(dev, None, slots=1, slot_bytes=64 << 20) as loop:
+        with pytest.raises(pg.PerftError):
+            loop.submit(g[0]["words"], 0)                 # depth 0
+        with pytest.raises(pg.PerftError):
+            loop.submit(g[0]["words"], 16)                # beyond the deepest table
+        with pytest.raises(pg.PerftError):
+            loop.submit([0] * 6, 3, 77)                   # no kings: refused before it can join a batch
+        good = loop.submit(g[0]["words"], 3, 78)
+        got = []
+        while not got:
+            got = loop.poll()
+        assert got == [(good, 78, g[0]["perft"][3], 3, pg.PG_OK, got[0][5])]
+        assert loop.stats()["submitted"] == 1
 
 
 def test_served_batch_overflow_finishes_in_halves(dev, pg):
@@ -81,3 +122,15 @@ def test_server_rejects_misuse(dev, pg):
     with pg.TranspositionTable(dev, 1 << 30) as tt:
         with pytest.raises(pg.PerftError):
             tt.perft(pg.pos_from_words(bad[0]), 3)
+
+
+def test_c_host_serves_a_stream(pg):
+    """perft_gpu_amd/perft_serve (plain C over the C ABI) serves 300 Poisson
+    queries with a shared table and checks each answer against pg_perft."""
+    import os
+    import subprocess
+    exe = os.path.join(os.path.dirname(pg.LIB_PATH), "perft_serve")
+    assert os.path.exists(exe), "perft_serve is built with the library (make -C perft_gpu_amd)"
+    out = subprocess.run([exe, "300", "2000", "1", "7"], capture_output=True, text=True, timeout=180)
+    assert out.returncode == 0, out.stdout + out.stderr
+    assert "served 300 queries" in out.stdout and "mismatches 0" in out.stdout
