@@ -115,6 +115,11 @@ typedef struct pg_ctx pg_ctx;
 /* initGPU (launcher.h:1506) + MoveGeneratorBitboard::init (:982):
  * selects the device, builds and uploads tables, reserves the arena. */
 int pg_init(int device, const pg_config *cfg, pg_ctx **ctx);
+/* Releases the context, and with it every transposition table and slot
+ * server created on it that is still alive (their handles become invalid).
+ * Serving loops (pg_serve_create) must be destroyed before, since each holds
+ * a slot server: the order is pg_serve_destroy, pg_server_destroy,
+ * pg_tt_destroy, pg_shutdown. */
 void pg_shutdown(pg_ctx *ctx);
 int pg_get_config(pg_ctx *ctx, pg_config *out);
 int pg_set_slider(pg_ctx *ctx, int slider);
@@ -154,6 +159,11 @@ int pg_device_make_move(pg_ctx *ctx, const pg_pos48 *pos, const pg_move *moves, 
                         pg_pos48 *out, pg_key128 *keys /* in: parent keys, out: child keys; may be NULL */);
 int pg_device_slider_attacks(pg_ctx *ctx, int slider, int rook, const uint8_t *squares,
                              const uint64_t *occupancy, uint32_t n, uint64_t *out);
+/* The level scan (ScanD, scan.cuh:329-375, the reference's exclusive prefix
+ * sum between levels, perft_bb.h:1092) on n u32 counts: offsets[i] = sum of
+ * counts[0..i), offsets[n] = the total; the counts are placed misalign (0-3)
+ * elements past a 16-byte boundary on the device. */
+int pg_device_scan(pg_ctx *ctx, const uint32_t *counts, uint64_t n, uint32_t misalign, uint64_t *offsets);
 
 /* ------------------------------------------------- transposition table */
 
@@ -172,8 +182,9 @@ int pg_tt_clear(pg_tt *tt);
  * same size and layout, and clears the table if the data do not match. */
 int pg_tt_save(pg_tt *tt, const char *path);
 int pg_tt_load(pg_tt *tt, const char *path);
-/* freeHashTables (launcher.h:322) */
-void pg_tt_destroy(pg_tt *tt);
+/* freeHashTables (launcher.h:322).  PG_EINVAL, and the table is kept, while
+ * a slot server (pg_server_create, pg_serve_create) still shares it. */
+int pg_tt_destroy(pg_tt *tt);
 /* allocated bytes and log2(slots) per remaining depth (-1: no table) */
 int pg_tt_info(pg_tt *tt, uint64_t *bytes, int32_t *bits_per_depth /* [16] */);
 /* PRINT_HASH_STATS counters (perft_bb.h:24-32), per remaining depth:

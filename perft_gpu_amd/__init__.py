@@ -138,11 +138,12 @@ def lib():
         "pg_device_slider_attacks": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.c_void_p,
                                                     ctypes.c_void_p, ctypes.c_uint32, ctypes.c_void_p]),
         "pg_device_copy": (ctypes.c_int, [ctypes.c_void_p, u64, ctypes.c_uint32, P(ctypes.c_double)]),
+        "pg_device_scan": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, u64, ctypes.c_uint32, ctypes.c_void_p]),
         "pg_device_issue_probe": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.c_uint32, P(ctypes.c_double),
                                                  P(u64), P(ctypes.c_uint32), P(ctypes.c_uint32)]),
         "pg_tt_create": (ctypes.c_int, [ctypes.c_void_p, u64, P(ctypes.c_void_p)]),
         "pg_tt_clear": (ctypes.c_int, [ctypes.c_void_p]),
-        "pg_tt_destroy": (None, [ctypes.c_void_p]),
+        "pg_tt_destroy": (ctypes.c_int, [ctypes.c_void_p]),
         "pg_tt_info": (ctypes.c_int, [ctypes.c_void_p, P(u64), ctypes.c_void_p]),
         "pg_tt_counters": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p]),
         "pg_tt_poison": (ctypes.c_int, [ctypes.c_void_p, u64]),
@@ -288,9 +289,20 @@ class Device:
         _check(L.pg_init(device, ctypes.byref(cfg), ctypes.byref(h)), "pg_init")
         self._h = h
         self.device = device
+        self._children = []   # tables, slot servers and serving loops made on this context
+
+    def _adopt(self, obj):
+        # the context keeps what was made on it until it closes it
+        self._children.append(obj)
 
     def close(self):
         if self._h:
+            # serving loops, then slot servers, then tables, then the context
+            for rank in (0, 1, 2):
+                for obj in self._children:
+                    if getattr(obj, "_close_rank", 2) == rank:
+                        obj.close()
+            self._children = []
             lib().pg_shutdown(self._h)
             self._h = None
 
@@ -379,6 +391,15 @@ class Device:
         return out
 
 
+    def scan(self, counts, misalign=0):
+        """The level scan on the device: exclusive prefix sums of u32 counts
+        (offsets[n] = total), with the counts `misalign` elements past a
+        16-byte boundary."""
+        c = np.ascontiguousarray(counts, dtype=np.uint32)
+        off = np.zeros(len(c) + 1, dtype=np.uint64)
+        _check(lib().pg_device_scan(self._h, _ptr(c), len(c), misalign, _ptr(off)), "pg_device_scan")
+        return off
+
     def copy_bandwidth(self, nbytes, iters=10):
         """Mean time (ms) and GB/s (read + write) of a plain nbytes copy."""
         ms = ctypes.c_double()
@@ -399,15 +420,18 @@ class Device:
 class TranspositionTable:
     """Device TT (pg_tt): setupHashTables128b (launcher.h:293), zeroed."""
 
+    _close_rank = 2
+
     def __init__(self, dev, nbytes):
         h = ctypes.c_void_p()
         _check(lib().pg_tt_create(dev._h, nbytes, ctypes.byref(h)), "pg_tt_create")
         self._h = h
         self.dev = dev
+        dev._adopt(self)
 
     def close(self):
         if self._h:
-            lib().pg_tt_destroy(self._h)
+            _check(lib().pg_tt_destroy(self._h), "pg_tt_destroy")
             self._h = None
 
     def __enter__(self):
@@ -460,7 +484,15 @@ def divide(dev, pos, depth, slider=SLIDER_AUTO, tt=None):
     if depth <= 1:
         return {m: 1 for m in moves}
     kids = np.array([make_move_host(pos, m)[0].words() for m in moves], dtype=np.uint64).reshape(-1, 6)
-    counts, _ = dev.perft_batch(kids, depth - 1, slider, True, 0, tt)
+    try:
+        counts, _ = dev.perft_batch(kids, depth - 1, slider, True, 0, tt)
+    except PerftError as e:
+        if e.code != PG_ENOMEM:
+            raise
+        # one child's levels do not fit a batch region: one perft per move,
+        # whose host-driven level loop splits levels to fit
+        counts = [(tt.perft(pos_from_words(k), depth - 1, slider) if tt is not None
+                   else dev.perft(pos_from_words(k), depth - 1, slider)) for k in kids]
     return {m: int(c) for m, c in zip(moves, counts)}
 
 

@@ -1,6 +1,7 @@
 // engine.h -- per-device perft context (the object behind pg_ctx).
 #pragma once
 #include <hip/hip_runtime.h>
+#include <vector>
 #include "../../include/perft_gpu.h"
 #include "tables.h"
 #include "tt.h"
@@ -30,6 +31,7 @@ int scratch_setup(int device, u8 **out);  // device address of the static scratc
 
 struct TTable;
 struct Slot;
+struct Server;
 
 struct Engine {
     int device = 0, cus = 0, bpc = 8, slider = SL_KOGGE;
@@ -70,6 +72,10 @@ struct Engine {
 
     // batched searches (graph.hip)
     Slot *batchSlot = nullptr;
+    // tables and slot servers created on this context; pg_shutdown releases
+    // those still alive
+    std::vector<TTable *> tts;
+    std::vector<Server *> servers;
 
     // transposition-table perft
     bool dedup = true;
@@ -85,13 +91,15 @@ void slot_forget_tt(Slot &sl, const TTable *tt);  // drop graphs that bake this 
 int batch_perft(Engine &e, Slot &sl, TTable *tt, const Pos *roots, u64 n, u32 depth, int v, int dedup, int bpc,
                 u64 *out, pg_batch_stats *bs);
 int perft_graph(Engine &e, const Pos &root, u32 depth, int v, u64 *out, bool *fits);
-struct Server;  // the object behind pg_server
+void tt_free(TTable *t);      // release a table (no users left)
+void server_free(Server *s);  // release a slot server
 int server_submit(Server &s, uint32_t slot, const pg_pos48 *pos, uint32_t n, uint32_t depth, int slider, int dedup,
                   int wgCap);
 
 // Device transposition table (the object behind pg_tt).
 struct TTable {
     Engine *eng = nullptr;
+    int users = 0;  // slot servers sharing this table
     u8 *mem = nullptr;
     size_t bytes = 0;
     TTView view{};

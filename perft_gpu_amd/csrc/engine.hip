@@ -409,6 +409,9 @@ void pg_shutdown(pg_ctx *c) {
     if (!c) return;
     (void)hipSetDevice(c->device);
     if (c->stream) (void)hipStreamSynchronize(c->stream);
+    // slot servers and tables still alive belong to this context
+    while (!c->servers.empty()) server_free(c->servers.back());
+    while (!c->tts.empty()) tt_free(c->tts.back());
     slot_delete(c->batchSlot);
     c->batchSlot = nullptr;
     if (c->arena) (void)hipFree(c->arena);
@@ -592,6 +595,22 @@ int pg_device_make_move(pg_ctx *c, const pg_pos48 *pos, const pg_move *moves, ui
     if (hipGetLastError() != hipSuccess) return PG_EHIP;
     PROBE_TRY(stage_out(c, out, dout, n * sizeof(Pos)));
     if (keys) PROBE_TRY(stage_out(c, keys, dk, n * sizeof(Key128)));
+    c->top = 0;
+    return PG_OK;
+}
+
+int pg_device_scan(pg_ctx *c, const uint32_t *counts, uint64_t n, uint32_t misalign, uint64_t *offsets) {
+    if (!c || !counts || !offsets || n == 0 || n > 0xFFFFFFFFULL || misalign > 3) return PG_EINVAL;
+    (void)hipSetDevice(c->device);
+    c->top = 0;
+    void *dc, *doff;
+    PROBE_TRY(stage_in(c, nullptr, (n + misalign) * sizeof(u32), &dc));
+    u32 *cnt = (u32 *)dc + misalign;  // the arena hands out 256-byte aligned blocks
+    if (hipMemcpyAsync(cnt, counts, n * sizeof(u32), hipMemcpyHostToDevice, c->stream) != hipSuccess) return PG_EHIP;
+    PROBE_TRY(stage_in(c, nullptr, (n + 1) * sizeof(u64), &doff));
+    u64 total = 0;
+    PROBE_TRY(c->scan(cnt, n, (u64 *)doff, &total));
+    PROBE_TRY(stage_out(c, offsets, doff, (n + 1) * sizeof(u64)));
     c->top = 0;
     return PG_OK;
 }

@@ -508,8 +508,10 @@ int pg_server_create(pg_ctx *ctx, uint32_t nslots, uint64_t bytes_per_slot, uint
     s->eng = e;
     s->tt = t;
     s->bytesPerSlot = bytes_per_slot;
+    if (t) t->users++;
+    e->servers.push_back(s);
     if (hipMalloc(&s->mem, (size_t)nslots * bytes_per_slot) != hipSuccess) {
-        delete s;
+        pg::server_free(s);
         return PG_ENOMEM;
     }
     s->slots.resize(nslots);
@@ -532,8 +534,21 @@ int pg_server_create(pg_ctx *ctx, uint32_t nslots, uint64_t bytes_per_slot, uint
 }
 
 void pg_server_destroy(pg_server *s) {
-    if (!s) return;
-    (void)hipSetDevice(s->eng->device);
+    if (s) pg::server_free(s);
+}
+
+}  // extern "C"
+
+void pg::server_free(Server *srv) {
+    pg_server *s = static_cast<pg_server *>(srv);
+    Engine *e = s->eng;
+    if (s->tt) s->tt->users--;
+    for (size_t i = 0; i < e->servers.size(); i++)
+        if (e->servers[i] == srv) {
+            e->servers.erase(e->servers.begin() + i);
+            break;
+        }
+    (void)hipSetDevice(e->device);
     for (auto &ss : s->slots) {
         if (ss.slot.stream) (void)hipStreamSynchronize(ss.slot.stream);
         hipStream_t st = ss.slot.stream;
@@ -545,6 +560,8 @@ void pg_server_destroy(pg_server *s) {
     if (s->mem) (void)hipFree(s->mem);
     delete s;
 }
+
+extern "C" {
 
 // Stage a batch on an idle slot and launch its graph; returns at once.
 int pg_server_submit(pg_server *s, uint32_t slot, const pg_pos48 *pos, uint32_t n, uint32_t depth, int slider,

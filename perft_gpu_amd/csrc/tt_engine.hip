@@ -220,6 +220,21 @@ static void tt_plan(u64 bytes, int bits[TT_MAX_DEPTH], u64 sizes[TT_MAX_DEPTH]) 
     }
 }
 
+void pg::tt_free(TTable *t) {
+    Engine *e = t->eng;
+    (void)hipSetDevice(e->device);
+    (void)hipStreamSynchronize(e->stream);
+    if (e->batchSlot) slot_forget_tt(*e->batchSlot, t);
+    for (size_t i = 0; i < e->tts.size(); i++)
+        if (e->tts[i] == t) {
+            e->tts.erase(e->tts.begin() + i);
+            break;
+        }
+    (void)hipFree(t->mem);
+    (void)hipFree(t->dCtr);
+    delete static_cast<pg_tt *>(t);
+}
+
 extern "C" {
 
 int pg_tt_create(pg_ctx *ctx, uint64_t bytes, pg_tt **out) {
@@ -257,6 +272,7 @@ int pg_tt_create(pg_ctx *ctx, uint64_t bytes, pg_tt **out) {
         delete t;
         return PG_EHIP;
     }
+    e->tts.push_back(t);
     *out = t;
     return PG_OK;
 }
@@ -269,14 +285,14 @@ int pg_tt_clear(pg_tt *t) {
     return PG_OK;
 }
 
-void pg_tt_destroy(pg_tt *t) {
-    if (!t) return;
-    (void)hipSetDevice(t->eng->device);
-    (void)hipStreamSynchronize(t->eng->stream);
-    if (t->eng->batchSlot) slot_forget_tt(*t->eng->batchSlot, t);
-    (void)hipFree(t->mem);
-    (void)hipFree(t->dCtr);
-    delete t;
+int pg_tt_destroy(pg_tt *t) {
+    if (!t) return PG_EINVAL;
+    if (t->users > 0) {
+        set_error("the table is shared by %d slot server(s): destroy them first", t->users);
+        return PG_EINVAL;
+    }
+    tt_free(t);
+    return PG_OK;
 }
 
 int pg_tt_info(pg_tt *t, uint64_t *bytes, int32_t *bits_per_depth /* [16] */) {

@@ -90,8 +90,9 @@ def policy(waiting, oldest, busy_long, idle_slots, region_bytes, est_bytes, **cf
 
 class ServeLoop:
     """pg_serve: the library's serving loop on one device context."""
+    _close_rank = 0
 
-    def __init__(self, dev, tt=None, slots=4, slot_bytes=0, max_batch=1024, long_depth=6, long_slots=0,
+    def __init__(self, dev, tt=None, slots=0, slot_bytes=0, max_batch=1024, long_depth=6, long_slots=0,
                  long_wg_per_cu=2, slider=SLIDER_KOGGE_STONE, dedup=True):
         L = _sigs()
         self.cfg = ServeConfig(slots, slot_bytes, max_batch, long_depth, long_slots, long_wg_per_cu, slider,
@@ -100,6 +101,7 @@ class ServeLoop:
         _check(L.pg_serve_create(dev._h, tt._h if tt is not None else None, ctypes.byref(self.cfg), ctypes.byref(h)),
                "pg_serve_create")
         self._h = h
+        dev._adopt(self)
         self._buf = (ServeResult * 4096)()
 
     def close(self):
@@ -187,6 +189,7 @@ def latency_summary(lat):
 class Server:
     """pg_server: `slots` slots, each with its own stream, graphs and HBM region
     (the slot-level calls the serving loop is built on)."""
+    _close_rank = 1
 
     def __init__(self, dev, slots=4, slot_bytes=None, max_roots=4096, tt=None, slider=SLIDER_KOGGE_STONE, dedup=True):
         L = _sigs()
@@ -199,6 +202,7 @@ class Server:
         _check(L.pg_server_create(dev._h, slots, slot_bytes, max_roots, tt._h if tt is not None else None,
                                   ctypes.byref(h)), "pg_server_create")
         self._h = h
+        dev._adopt(self)
         self.slots = slots
         self.max_roots = max_roots
         self.slider = slider

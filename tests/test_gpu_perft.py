@@ -180,3 +180,19 @@ def test_graph_path_overflow_is_reported(gdev, pg):
         assert gdev.perft(W.CPW["start"], 6, 0) == (1 << 64) - 1
     finally:
         gdev.set_count_bias(0)
+
+
+def test_divide_falls_back_when_a_child_does_not_fit_a_batch(dev, pg):
+    """Kiwipete divided at depth 7 on a 64 MiB arena: a depth-6 child's ply-4
+    level (millions of boards) does not fit one batch region, so divide()
+    runs one perft per move (the host-driven loop splits levels) and still
+    sums to the published total."""
+    with pg.Device(0, arena_bytes=64 << 20) as small:
+        pos = pg.read_fen_string(W.CPW["kiwipete"])
+        kid, _ = pg.make_move_host(pos, pg.generate_moves_host(pos)[0])
+        with pytest.raises(pg.PerftError) as e:
+            small.perft_batch(pg.positions_array([kid]), 6, 0, True)
+        assert e.value.code == pg.PG_ENOMEM
+        print("divide kiwipete d7 on 64 MiB", flush=True)
+        div = pg.divide(small, W.CPW["kiwipete"], 7)
+    assert len(div) == 48 and sum(div.values()) == 374190009323 == dev.perft(W.CPW["kiwipete"], 7, 0)

@@ -134,3 +134,32 @@ def test_c_host_serves_a_stream(pg):
     out = subprocess.run([exe, "300", "2000", "1", "7"], capture_output=True, text=True, timeout=180)
     assert out.returncode == 0, out.stdout + out.stderr
     assert "served 300 queries" in out.stdout and "mismatches 0" in out.stdout
+
+
+def test_handle_lifetimes(pg):
+    """A table shared by a slot server cannot be destroyed until the server
+    is; closing a context releases its serving loops, servers and tables
+    (through the C ABI: pg_shutdown releases what is still alive)."""
+    import ctypes
+    from perft_gpu_amd.serve import Server, ServeLoop
+    with pg.Device(0, arena_bytes=1 << 30) as d:
+        tt = pg.TranspositionTable(d, 1 << 30)
+        srv = Server(d, slots=1, slot_bytes=64 << 20, tt=tt)
+        with pytest.raises(pg.PerftError) as e:
+            tt.close()
+        assert e.value.code == pg.PG_EINVAL and tt._h
+        srv.close()
+        tt.close()
+        # left open on purpose: the context closes them in order
+        tt2 = pg.TranspositionTable(d, 1 << 30)
+        ServeLoop(d, tt2, slots=1, slot_bytes=64 << 20)
+        Server(d, slots=1, slot_bytes=64 << 20, tt=tt2)
+    assert tt2._h is None
+    # C ABI: a context shut down with a table and a server still alive
+    L = pg.lib()
+    ctx, t, s = ctypes.c_void_p(), ctypes.c_void_p(), ctypes.c_void_p()
+    assert L.pg_init(0, None, ctypes.byref(ctx)) == 0
+    assert L.pg_tt_create(ctx, 1 << 30, ctypes.byref(t)) == 0
+    assert L.pg_server_create(ctx, 1, 64 << 20, 8, t, ctypes.byref(s)) == 0
+    assert L.pg_tt_destroy(t) == pg.PG_EINVAL
+    L.pg_shutdown(ctx)
