@@ -142,7 +142,8 @@ def leg_tt(pg, dev, sizes=(16, 64), depths=(9, 10)):
                     raise SystemExit("MISMATCH TT perft(%d)" % d)
                 st = dev.last_stats()
                 out.append({"tt_gb": gb, "depth": d, "perft": n, "s": dt, "effective_leaves_per_s": n / dt,
-                            "peak_arena_gb": st["peak_arena"] / 1e9})
+                            "peak_arena_gb": st["peak_arena"] / 1e9, "boards_per_ply": st["nodes"][1:d],
+                            "leaf_items": st["leaf_items"], "splits": st["splits"]})
                 log("tt %d GB d%d %.3f s" % (gb, d, dt))
             c = tt.counters()
             out[-1]["counters_by_remaining_depth"] = {
@@ -448,9 +449,10 @@ def main():
     # leaf-kernel launch with HIP events on its launch stream (the host-driven
     # level loop, which launches the same fused leaf as a kernel of its own)
     acc = {"launches": 0, "ms": 0.0, "items": 0, "parents": 0}
-    with pg.Device(0, arena_bytes=48 << 30, timing=True) as tdev:
-        run_step(tdev, jobs, best)
-        run_step(tdev, jobs, best, acc)
+    if args.steps > 0:
+        with pg.Device(0, arena_bytes=48 << 30, timing=True) as tdev:
+            run_step(tdev, jobs, best)
+            run_step(tdev, jobs, best, acc)
 
     launches = max(1, acc["launches"])
     avg_ms = acc["ms"] / launches
