@@ -413,7 +413,8 @@ def main():
     # the serving leg runs four slot streams next to the context's and the
     # runtime's null stream: give each its own hardware queue (the HIP
     # runtime reads this once, at its initialization, which is still ahead)
-    os.environ.setdefault("GPU_MAX_HW_QUEUES", "8")
+    if int(os.environ.get("GPU_MAX_HW_QUEUES", "4") or 4) < 8:
+        os.environ["GPU_MAX_HW_QUEUES"] = "8"
 
     import perft_gpu_amd as pg
     # the counter passes run in child processes before this one touches the GPU

@@ -8,7 +8,7 @@
  * loop with pg_serve_poll until every query is answered, then checks each
  * answer against pg_perft of the same query alone.  Prints throughput and
  * p50/p99 latency; exits non-zero on any error or mismatch. */
-#define _POSIX_C_SOURCE 199309L
+#define _POSIX_C_SOURCE 200112L
 #include <math.h>
 #include <stdint.h>
 #include <stdio.h>
@@ -54,6 +54,12 @@ int main(int argc, char **argv) {
         t += -log(1.0 - uniform()) / rate;
         arrive[i] = t;
     }
+    /* four slot streams beside the context's stream and the runtime's null
+     * stream: with fewer hardware queues than that, short batches queue
+     * behind long ones on a shared queue.  The HIP runtime reads this once,
+     * at its initialization (the pg_init below). */
+    const char *hwq = getenv("GPU_MAX_HW_QUEUES");
+    if (!hwq || atoi(hwq) < 8) setenv("GPU_MAX_HW_QUEUES", "8", 1);
     pg_ctx *ctx = NULL;
     if (pg_init(0, NULL, &ctx) != PG_OK) {
         fprintf(stderr, "pg_init failed: %s\n", pg_last_error());
