@@ -329,9 +329,11 @@ def leg_serve(pg, dev):
                "max_busy_slots": r["max_busy_slots"], "max_long_slots": r["max_long_slots"],
                "plain_leaves_per_s": leaves / float(took[idx].sum())}
         out.update(latency_summary(r["latency"]))
-        lat, dq = np.asarray(r["latency"]), depths[idx]
-        out["by_depth"] = {int(d): dict(latency_summary(lat[dq == d]), queries=int((dq == d).sum()))
+        lat, dq, qd = np.asarray(r["latency"]), depths[idx], np.asarray(r["queued"]) * 1e3
+        out["by_depth"] = {int(d): dict(latency_summary(lat[dq == d]), queries=int((dq == d).sum()),
+                                        queued_p99_ms=float(np.percentile(qd[dq == d], 99)))
                            for d in sorted(set(int(x) for x in dq))}
+        out["submit_ms_max"] = r["stats"]["submit_ms_max"]
         log("serve rate %s: p50 %.2f ms p99 %.2f ms %.1f G leaves/s" % (rate, out["p50_ms"], out["p99_ms"],
                                                                       out["leaves_per_s"] / 1e9))
         return out

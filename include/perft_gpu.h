@@ -312,6 +312,7 @@ typedef struct pg_serve_result {
     uint32_t depth;
     int status;         /* PG_OK or the error of the query's batch */
     double latency_ms;  /* submit to completion seen by pg_serve_poll */
+    double queued_ms;   /* of which waiting before its batch was launched */
 } pg_serve_result;
 typedef struct pg_serve_stats {
     uint64_t submitted, completed, batches, launches, splits, leaves;
@@ -319,6 +320,8 @@ typedef struct pg_serve_stats {
     double p50_ms, p99_ms, mean_ms, max_ms;  /* latency of the completed queries */
     double elapsed_ms;                       /* first submit to last completion */
     double est_bytes_per_query[16];          /* current admission estimate per depth */
+    double submit_ms_max, submit_ms_total;   /* host time spent launching batches */
+    uint64_t graphs;                         /* graphs instantiated by the slots so far */
 } pg_serve_stats;
 typedef struct pg_serve pg_serve;
 int pg_serve_create(pg_ctx *ctx, pg_tt *tt /* may be NULL */, const pg_serve_config *cfg, pg_serve **out);

@@ -74,8 +74,8 @@ static __global__ __launch_bounds__(256) void kd_root_probe(Plan *P, TTView tt, 
         const bool probe = D >= 2 && tt.base[D] && L.k;
         const bool hit = probe && tt_probe(tt, D, L.k[i], &v);
         if (ctr) {
-            wave_count(&ctr->c[D][0], probe);
-            wave_count(&ctr->c[D][1], hit);
+            ctr_count(ctr, D, 0, probe);
+            ctr_count(ctr, D, 1, hit);
         }
         // the roots are left untouched so that a rerun after an overflow sees
         // the same input; a hit is recorded in state/perft and expands to nothing
@@ -90,7 +90,7 @@ static __global__ __launch_bounds__(256) void kd_root_store(Plan *P, TTView tt, 
     if (P->overflow || P->bias) return;  // a biased test total is not a perft
     PG_GRID_LOOP(i, L.n) {
         const bool st = D >= 2 && L.k && L.state[i] == ST_LIVE && tt_store(tt, D, L.k[i], L.perft[i]);
-        if (ctr) wave_count(&ctr->c[D][2], st);
+        if (ctr) ctr_count(ctr, D, 2, st);
     }
 }
 
@@ -246,8 +246,8 @@ __global__ __launch_bounds__(wg_size<V>()) void kd_expand(Plan *P, int l, TTView
             const bool probe = rc >= 2 && tt.base[rc];
             hit = probe && tt_probe(tt, rc, h, &v);
             if (ctr) {
-                wave_count(&ctr->c[rc][0], probe);
-                wave_count(&ctr->c[rc][1], hit);
+                ctr_count(ctr, rc, 0, probe);
+                ctr_count(ctr, rc, 1, hit);
             }
         } else {
             make_any(p, L.mv[i], t);
@@ -288,7 +288,7 @@ __device__ __forceinline__ void kd_dedup_finish(Plan *P, const DLevel &L, const 
             cur = atomicCAS(&L.set[s], 0u, (u32)i + 1);
         }
     }
-    if (ctr) wave_count(&ctr->c[rc][3], isDup);
+    if (ctr) ctr_count(ctr, rc, 3, isDup);
     if (isDup) {
         C.state[i] = ST_DUP;
         C.cnt[i] = 0;
@@ -354,7 +354,7 @@ static __global__ __launch_bounds__(256) void kd_resolve(Plan *P, int l, TTView 
         bool stored = false;
         if (st == ST_DUP) C.perft[i] = C.perft[C.dup[i]];
         else if (st == ST_LIVE && C.k && rc >= 2) stored = tt_store(tt, rc, C.k[i], C.perft[i]);
-        if (ctr) wave_count(&ctr->c[rc][2], stored);
+        if (ctr) ctr_count(ctr, rc, 2, stored);
     }
 }
 

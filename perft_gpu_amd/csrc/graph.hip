@@ -464,6 +464,7 @@ struct ServerSlot {
     int v = 0, dedup = 0, wgCap = 0;
     std::vector<u64> order;
     u64 *hOut = nullptr;  // pinned results, slot capacity
+    u8 *hIn = nullptr;    // pinned roots then keys, slot capacity (a pageable source would make the copy block)
     u64 cap = 0;
     DLevel L0{};
     size_t top0 = 0;
@@ -523,6 +524,8 @@ int pg_server_create(pg_ctx *ctx, uint32_t nslots, uint64_t bytes_per_slot, uint
         if (!rc) rc = slot_init(*e, ss.slot, s->mem + (size_t)i * bytes_per_slot, bytes_per_slot, st);
         if (!rc && hipEventCreateWithFlags(&ss.done, hipEventDisableTiming) != hipSuccess) rc = PG_EHIP;
         if (!rc && hipHostMalloc(&ss.hOut, (size_t)max_roots * 8) != hipSuccess) rc = PG_ENOMEM;
+        if (!rc && hipHostMalloc(&ss.hIn, (size_t)max_roots * (sizeof(Pos) + sizeof(Key128))) != hipSuccess)
+            rc = PG_ENOMEM;
         if (rc) {
             pg_server_destroy(s);
             return rc;
@@ -556,6 +559,7 @@ void pg::server_free(Server *srv) {
         if (st) (void)hipStreamDestroy(st);
         if (ss.done) (void)hipEventDestroy(ss.done);
         if (ss.hOut) (void)hipHostFree(ss.hOut);
+        if (ss.hIn) (void)hipHostFree(ss.hIn);
     }
     if (s->mem) (void)hipFree(s->mem);
     delete s;
@@ -600,8 +604,8 @@ int pg::server_submit(Server &srv, uint32_t slot, const pg_pos48 *pos, uint32_t 
     ss.order.resize(n);
     std::iota(ss.order.begin(), ss.order.end(), 0);
     std::stable_sort(ss.order.begin(), ss.order.end(), [&](u64 a, u64 b) { return pos_side(roots[a]) < pos_side(roots[b]); });
-    std::vector<Pos> hb(n);
-    std::vector<Key128> hk(n);
+    Pos *hb = reinterpret_cast<Pos *>(ss.hIn);
+    Key128 *hk = reinterpret_cast<Key128 *>(ss.hIn + (size_t)n * sizeof(Pos));
     for (uint32_t i = 0; i < n; i++) {
         hb[i] = roots[ss.order[i]];
         hk[i] = zobrist128(hb[i], ht->view());
@@ -623,8 +627,8 @@ int pg::server_submit(Server &srv, uint32_t slot, const pg_pos48 *pos, uint32_t 
     L0.state = (u8 *)take(n);
     L0.dup = (u32 *)take(n * 4);
     if (!L0.dup) return PG_ENOMEM;
-    HIP_TRY(hipMemcpyAsync(L0.b, hb.data(), n * sizeof(Pos), hipMemcpyHostToDevice, sl.stream));
-    HIP_TRY(hipMemcpyAsync(L0.k, hk.data(), n * sizeof(Key128), hipMemcpyHostToDevice, sl.stream));
+    HIP_TRY(hipMemcpyAsync(L0.b, hb, n * sizeof(Pos), hipMemcpyHostToDevice, sl.stream));
+    HIP_TRY(hipMemcpyAsync(L0.k, hk, n * sizeof(Key128), hipMemcpyHostToDevice, sl.stream));
     GraphShape sh{(int)depth, v, 1, dedup ? 1 : 0, e.bpc, s->tt, 0, wgCap, grid_limit(e, n, (int)depth)};
     hipGraphExec_t exec;
     int rc = get_graph(e, sl, sh, &exec);

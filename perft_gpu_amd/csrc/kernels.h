@@ -311,7 +311,7 @@ __device__ __forceinline__ void leaf_tiles(const Pos *__restrict__ boards, const
     __shared__ u32 snext;
     const int tid = threadIdx.x;
     const Tables &tl = t;
-    u64 s = 0;
+    u64 s = 0, itemSum = 0;
     for (u64 tile = blockIdx.x; tile * LEAF_WG < n; tile += gridDim.x) {
         const u64 i = tile * LEAF_WG + tid;
         const bool live = i < n;
@@ -321,7 +321,7 @@ __device__ __forceinline__ void leaf_tiles(const Pos *__restrict__ boards, const
         u64 total64;
         const u32 pos = (u32)block_excl_scan<LEAF_WG>(c, &total64);
         const u32 total = (u32)total64;
-        if (tid == 0 && items) atomicAdd(items, total64);
+        itemSum += total64;
         u32 start = 0;
         while (start < total) {
             const u32 end = start + LEAF_CAP;
@@ -354,6 +354,7 @@ __device__ __forceinline__ void leaf_tiles(const Pos *__restrict__ boards, const
         }
         __syncthreads();
     }
+    if (tid == 0 && items && itemSum) atomicAdd(items, itemSum);
     if (!PER) {
         s = wave_sum(s);
         if ((tid & 63) == 0 && s) add_checked(acc, s, ovf);

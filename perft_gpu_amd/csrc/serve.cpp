@@ -51,11 +51,13 @@ struct Query {
     pg_pos48 pos;
     uint32_t depth;
     double arrived;
+    double admitted;  // batch launched
 };
 
 struct SlotState {
     bool busy = false;
     uint32_t depth = 0;
+    uint64_t graphs = 0;        // graphs the slot holds (as of its last finished batch)
     std::vector<uint64_t> ids;  // queries of the running batch, submission order
 };
 
@@ -168,6 +170,7 @@ int pg_serve_submit(pg_serve *s, const pg_pos48 *pos, uint32_t depth, uint64_t t
     q.pos = *pos;
     q.depth = depth;
     q.arrived = now_ms();
+    q.admitted = q.arrived;
     if (s->firstSubmit < 0) s->firstSubmit = q.arrived;
     s->queries.emplace(q.id, q);
     s->waiting[depth].push_back(q.id);
@@ -199,12 +202,15 @@ int pg_serve_poll(pg_serve *s, pg_serve_result *out, uint32_t max, uint32_t *n) 
             res.status = r == 1 ? PG_OK : r;
             res.perft = r == 1 ? s->results[j] : 0;
             res.latency_ms = t - it->second.arrived;
+            res.queued_ms = it->second.admitted - it->second.arrived;
             s->latencies.push_back(res.latency_ms);
             if (r == 1) s->st.leaves += res.perft;
             s->done.push_back(res);
             s->queries.erase(it);
         }
         s->st.completed += sl.ids.size();
+        s->st.graphs += bs.graphs_cached > sl.graphs ? bs.graphs_cached - sl.graphs : 0;
+        sl.graphs = std::max(sl.graphs, bs.graphs_cached);
         s->st.launches += bs.launches;
         s->st.splits += bs.splits;
         s->lastDone = t;
@@ -239,15 +245,21 @@ int pg_serve_poll(pg_serve *s, pg_serve_result *out, uint32_t max, uint32_t *n) 
         if (pg_serve_policy(w, oldest, busyLong, idle, &s->cfg, s->regionBytes, s->est, &depth, &count) != 1) break;
         SlotState &sl = s->slots[slot];
         sl.ids.clear();
+        const double admitted = now_ms();
         for (uint32_t j = 0; j < count; j++) {
             const uint64_t id = s->waiting[depth].front();
             s->waiting[depth].pop_front();
             sl.ids.push_back(id);
-            s->stage[j] = s->queries[id].pos;
+            Query &q = s->queries[id];
+            q.admitted = admitted;
+            s->stage[j] = q.pos;
         }
         const uint32_t cap = depth >= s->cfg.long_depth ? s->cfg.long_wg_per_cu : 0;
         const int rc = pg_server_submit_capped(s->srv, slot, s->stage.data(), count, depth, s->cfg.slider, s->cfg.dedup,
                                                cap);
+        const double took = now_ms() - admitted;
+        s->st.submit_ms_max = std::max(s->st.submit_ms_max, took);
+        s->st.submit_ms_total += took;
         if (rc != PG_OK) {
             // the batch could not start: its queries finish with the error
             const double t = now_ms();
@@ -259,6 +271,7 @@ int pg_serve_poll(pg_serve *s, pg_serve_result *out, uint32_t max, uint32_t *n) 
                 res.depth = depth;
                 res.status = rc;
                 res.latency_ms = t - it->second.arrived;
+                res.queued_ms = res.latency_ms;
                 s->done.push_back(res);
                 s->queries.erase(it);
             }

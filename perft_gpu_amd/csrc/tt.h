@@ -74,14 +74,21 @@ __device__ __forceinline__ bool tt_store(const TTView &tt, int r, const Key128 &
 }
 
 
-// counters: [r][0] probes, [r][1] hits, [r][2] stores, [r][3] duplicates
+// counters: [copy][r][0] probes, [r][1] hits, [r][2] stores, [r][3]
+// duplicates.  Work-groups add into copy blockIdx % TT_CTR_COPIES (512 bytes
+// apart), so the waves of a large kernel do not all update the same few
+// addresses -- atomics to one address serialize at its L2 channel and delay
+// every other access that channel serves, including those of kernels on
+// other streams.  Readers sum the copies.
+constexpr int TT_CTR_COPIES = 64;
 struct TTCounters {
-    unsigned long long c[TT_MAX_DEPTH][4];
+    unsigned long long c[TT_CTR_COPIES][TT_MAX_DEPTH][4];
 };
 
-__device__ __forceinline__ void wave_count(unsigned long long *dst, bool pred) {
+__device__ __forceinline__ void ctr_count(TTCounters *ctr, int r, int k, bool pred) {
     const u64 m = __ballot(pred);
-    if ((threadIdx.x & 63) == __ffsll((long long)__ballot(1)) - 1 && m) atomicAdd(dst, (unsigned long long)__popcll(m));
+    if ((threadIdx.x & 63) == __ffsll((long long)__ballot(1)) - 1 && m)
+        atomicAdd(&ctr->c[blockIdx.x % TT_CTR_COPIES][r][k], (unsigned long long)__popcll(m));
 }
 
 // child = make(parent, move) with the incremental key; probe the table of the
@@ -103,8 +110,8 @@ __global__ __launch_bounds__(wg_size<V>()) void k_expand_tt(const Pos *__restric
         const bool probe = rc >= 2 && tt.base[rc];
         const bool hit = probe && tt_probe(tt, rc, h, &v);
         if (ctr) {
-            wave_count(&ctr->c[rc][0], probe);
-            wave_count(&ctr->c[rc][1], hit);
+            ctr_count(ctr, rc, 0, probe);
+            ctr_count(ctr, rc, 1, hit);
         }
         if (hit) {
             p.white = 0;
@@ -153,7 +160,7 @@ __device__ __forceinline__ void dedup_finish(u64 i, bool live, u64 s, u32 cur, P
             cur = atomicCAS(&set[s], 0u, (u32)i + 1);
         }
     }
-    if (ctr) wave_count(&ctr->c[rc][3], isDup);
+    if (ctr) ctr_count(ctr, rc, 3, isDup);
     if (isDup) {
         state[i] = ST_DUP;
         cnt[i] = 0;
@@ -189,7 +196,7 @@ static __global__ __launch_bounds__(256) void k_resolve_store(const Key128 *__re
         bool stored = false;
         if (st == ST_DUP) cperft[i] = cperft[dup[i]];
         else if (st == ST_LIVE && rc >= 2) stored = tt_store(tt, rc, ckeys[i], cperft[i]);
-        if (ctr) wave_count(&ctr->c[rc][2], stored);
+        if (ctr) ctr_count(ctr, rc, 2, stored);
     }
 }
 

@@ -306,7 +306,14 @@ int pg_tt_info(pg_tt *t, uint64_t *bytes, int32_t *bits_per_depth /* [16] */) {
 int pg_tt_counters(pg_tt *t, uint64_t *out /* [16][4] */) {
     if (!t || !out) return PG_EINVAL;
     (void)hipSetDevice(t->eng->device);
-    if (hipMemcpy(out, t->dCtr, sizeof(TTCounters), hipMemcpyDeviceToHost) != hipSuccess) return PG_EHIP;
+    std::vector<TTCounters> h(1);
+    if (hipMemcpy(h.data(), t->dCtr, sizeof(TTCounters), hipMemcpyDeviceToHost) != hipSuccess) return PG_EHIP;
+    for (int r = 0; r < TT_MAX_DEPTH; r++)
+        for (int k = 0; k < 4; k++) {
+            uint64_t sum = 0;
+            for (int c = 0; c < TT_CTR_COPIES; c++) sum += h[0].c[c][r][k];
+            out[r * 4 + k] = sum;
+        }
     return PG_OK;
 }
 

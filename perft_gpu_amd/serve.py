@@ -27,7 +27,7 @@ class ServeConfig(ctypes.Structure):
 
 class ServeResult(ctypes.Structure):
     _fields_ = [("id", u64), ("tag", u64), ("perft", u64), ("depth", ctypes.c_uint32), ("status", ctypes.c_int),
-                ("latency_ms", ctypes.c_double)]
+                ("latency_ms", ctypes.c_double), ("queued_ms", ctypes.c_double)]
 
 
 class ServeStats(ctypes.Structure):
@@ -36,7 +36,8 @@ class ServeStats(ctypes.Structure):
                 ("max_busy_slots", ctypes.c_uint32), ("max_long_slots", ctypes.c_uint32),
                 ("p50_ms", ctypes.c_double), ("p99_ms", ctypes.c_double), ("mean_ms", ctypes.c_double),
                 ("max_ms", ctypes.c_double), ("elapsed_ms", ctypes.c_double),
-                ("est_bytes_per_query", ctypes.c_double * 16)]
+                ("est_bytes_per_query", ctypes.c_double * 16), ("submit_ms_max", ctypes.c_double),
+                ("submit_ms_total", ctypes.c_double), ("graphs", u64)]
 
     def as_dict(self):
         d = {n: getattr(self, n) for n, _ in self._fields_ if n != "est_bytes_per_query"}
@@ -125,12 +126,13 @@ class ServeLoop:
 
     def poll(self):
         """Runs the loop once; returns the queries finished since the last poll
-        as (id, tag, perft, depth, status, latency_ms) tuples."""
+        as (id, tag, perft, depth, status, latency_ms, queued_ms) tuples."""
         out = []
         while True:
             n = ctypes.c_uint32()
             _check(lib().pg_serve_poll(self._h, self._buf, len(self._buf), ctypes.byref(n)), "pg_serve_poll")
-            out.extend((r.id, r.tag, r.perft, r.depth, r.status, r.latency_ms) for r in self._buf[:n.value])
+            out.extend((r.id, r.tag, r.perft, r.depth, r.status, r.latency_ms, r.queued_ms)
+                       for r in self._buf[:n.value])
             if n.value < len(self._buf):
                 return out
 
@@ -149,13 +151,18 @@ def serve(loop, positions, depths, arrivals):
     """Feeds query i to the loop at arrivals[i] seconds from the start and
     polls until every query is answered.  Latency counts from the scheduled
     arrival (the loop's own submit-to-done latency plus any delay before the
-    submit).  Returns results, latencies (s), elapsed time and loop stats."""
+    submit); `queued` is the part of it before the query's batch was
+    launched.  Returns results, latencies (s), elapsed time, this stream's
+    batches and the loop's stats (max_busy_slots and max_long_slots are
+    maxima over the loop's lifetime)."""
     positions = positions_array(positions)
     n = len(positions)
     results = np.zeros(n, dtype=np.uint64)
     latency = np.zeros(n)
+    queued = np.zeros(n)
     late = np.zeros(n)
     pos = [Pos48(*[int(x) for x in w]) for w in positions]
+    st0 = loop.stats()
     finished, nxt = 0, 0
     t0 = time.perf_counter()
     while finished < n:
@@ -164,10 +171,11 @@ def serve(loop, positions, depths, arrivals):
             late[nxt] = now - arrivals[nxt]
             loop.submit(pos[nxt], int(depths[nxt]), nxt)
             nxt += 1
-        for qid, tag, perft, depth, status, lat_ms in loop.poll():
+        for qid, tag, perft, depth, status, lat_ms, q_ms in loop.poll():
             _check(status, "served query %d" % tag)
             results[tag] = perft
             latency[tag] = late[tag] + lat_ms * 1e-3
+            queued[tag] = late[tag] + q_ms * 1e-3
             finished += 1
         if finished == nxt and nxt < n:
             wait = arrivals[nxt] - (time.perf_counter() - t0)
@@ -175,8 +183,10 @@ def serve(loop, positions, depths, arrivals):
                 time.sleep(min(wait, 0.0005))
     elapsed = time.perf_counter() - t0
     st = loop.stats()
-    return {"results": results, "latency": latency, "elapsed": elapsed, "batches": st["batches"],
-            "mean_batch": st["completed"] / max(1, st["batches"]), "max_busy_slots": st["max_busy_slots"],
+    # the loop's counters run over its lifetime: this stream's share
+    batches = st["batches"] - st0["batches"]
+    return {"results": results, "latency": latency, "queued": queued, "elapsed": elapsed, "batches": batches,
+            "mean_batch": (st["completed"] - st0["completed"]) / max(1, batches), "max_busy_slots": st["max_busy_slots"],
             "max_long_slots": st["max_long_slots"], "splits": st["splits"], "stats": st}
 
 

@@ -76,7 +76,8 @@ def test_serve_rejects_bad_queries(dev, pg):
         got = []
         while not got:
             got = loop.poll()
-        assert got == [(good, 78, g[0]["perft"][3], 3, pg.PG_OK, got[0][5])]
+        assert got == [(good, 78, g[0]["perft"][3], 3, pg.PG_OK, got[0][5], got[0][6])]
+        assert 0 <= got[0][6] <= got[0][5]
         assert loop.stats()["submitted"] == 1
 
 
