@@ -222,7 +222,8 @@ def leg_counters(pg, dev=None):
     exe = shutil.which("rocprofv3")
     if exe is None:
         return {"measured": False, "reason": "rocprofv3 not found"}
-    passes = [["SQ_INSTS_VALU", "SQ_WAVES", "GRBM_GUI_ACTIVE", "FETCH_SIZE"],
+    passes = [["SQ_INSTS_VALU", "SQ_WAVES", "GRBM_GUI_ACTIVE", "FETCH_SIZE", "SQ_ACTIVE_INST_VALU2",
+               "SQ_THREAD_CYCLES_VALU", "SQ_ACTIVE_INST_VALU"],
               ["WRITE_SIZE", "SQ_INSTS_LDS", "SQ_LDS_BANK_CONFLICT", "SQ_INSTS_SALU"]]
     child = [sys.executable, os.path.join(ROOT, "bench.py"), "--steps", "1", "--warmup", "0", "--no-cpu",
              "--legs", "calib", "--variant", "0"]
@@ -270,6 +271,12 @@ def leg_counters(pg, dev=None):
            # the per-lane program length
            "valu_insts_per_board": L["SQ_INSTS_VALU"] * 64.0 / boards,
            "valu_wave_insts_per_cu_per_cycle": L["SQ_INSTS_VALU"] / (cus * cycles),
+           # SQ_ACTIVE_INST_VALU counts quad-cycles per wave: one per wave64
+           # instruction, so each SIMD's VALU is busy this fraction of the
+           # time (above 1 by the quad-cycles that issued two instructions)
+           "valu_busy_per_simd": L.get("SQ_ACTIVE_INST_VALU", 0) * 4.0 / (cus * 4 * cycles),
+           "dual_issue_fraction": L.get("SQ_ACTIVE_INST_VALU2", 0) / L["SQ_INSTS_VALU"],
+           "lane_utilization": L.get("SQ_THREAD_CYCLES_VALU", 0) / (64.0 * max(1.0, L.get("SQ_ACTIVE_INST_VALU", 0))),
            "lds_insts": L.get("SQ_INSTS_LDS"), "lds_bank_conflict_cycles": L.get("SQ_LDS_BANK_CONFLICT"),
            "salu_insts": L.get("SQ_INSTS_SALU"), "waves": L.get("SQ_WAVES"),
            "hbm_bytes_per_launch": traffic, "algorithmic_bytes_per_launch": 52.0 * tot["parents"] / tot["launches"],
@@ -278,16 +285,18 @@ def leg_counters(pg, dev=None):
                                 "write_size_kib_per_dispatch": C["WRITE_SIZE"] / nc,
                                 "fetch_factor": ff, "write_factor": wf}}
     probes = {}
-    for name in ("k_issue<0>", "k_issue<1>", "k_issue<2>", "k_issue<3>"):
-        k = _find(sums, name)
+    for kind, iname in pg.Device.ISSUE_KINDS.items():
+        k = _find(sums, "k_issue<%d>" % kind)
         if k and sums[k].get("GRBM_GUI_ACTIVE"):
-            probes[name] = {"valu_wave_insts": sums[k]["SQ_INSTS_VALU"], "dispatches": disp[k],
-                            "wave_insts_per_cu_per_cycle": sums[k]["SQ_INSTS_VALU"]
-                            / (cus * sums[k]["GRBM_GUI_ACTIVE"] / 8.0)}
+            probes[iname] = {"valu_wave_insts": sums[k]["SQ_INSTS_VALU"], "dispatches": disp[k],
+                             "wave_insts_per_cu_per_cycle": sums[k]["SQ_INSTS_VALU"]
+                             / (cus * sums[k]["GRBM_GUI_ACTIVE"] / 8.0),
+                             "dual_issue_fraction": sums[k].get("SQ_ACTIVE_INST_VALU2", 0) / sums[k]["SQ_INSTS_VALU"]}
     res["issue_probes"] = probes
-    log("counters: %.0f VALU instructions per board, %.3f per CU per cycle, %.1f B HBM per parent"
-        % (res["valu_insts_per_board"], res["valu_wave_insts_per_cu_per_cycle"],
-           traffic * tot["launches"] / max(1, tot["parents"])))
+    log("counters: %.0f VALU instructions per board, %.3f per CU per cycle (dual issue %.1f%%, lanes active %.1f%%), "
+        "%.1f B HBM per parent" % (res["valu_insts_per_board"], res["valu_wave_insts_per_cu_per_cycle"],
+                                   100 * res["dual_issue_fraction"], 100 * res["lane_utilization"],
+                                   traffic * tot["launches"] / max(1, tot["parents"])))
     return res
 
 

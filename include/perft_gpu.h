@@ -95,6 +95,12 @@ int pg_host_count_moves(const pg_pos48 *pos, uint32_t n, int slider, uint32_t *o
  * the children of board i follow those of board i-1 in out (at most cap of
  * them in all; PG_ENOMEM beyond), counts[i] = their number. */
 int pg_host_children(const pg_pos48 *pos, uint32_t n, pg_pos48 *out, uint64_t cap, uint32_t *counts);
+/* perft_bb (perft_bb.h:79-101): the reference's CPU perft -- one host
+ * thread, generateBoards-style recursion with the legal-move count at depth
+ * 1, fixed-shift fancy magics as the reference's CPU build uses.  A separate
+ * entry point for callers that want the CPU path; no device entry point
+ * ever calls it. */
+int pg_cpu_perft(const pg_pos48 *pos, uint32_t depth, uint64_t *out);
 /* MoveGeneratorBitboard::makeMove<chance, updateHash> (:3575); key may be NULL. */
 int pg_host_make_move(pg_pos48 *pos, pg_move move, pg_key128 *key);
 /* Slider lookups of one variant on the host tables (the device copies are
@@ -210,10 +216,22 @@ typedef struct pg_deep_stats {
     uint64_t device_calls;  /* device perft calls made */
     uint64_t host_nodes;    /* host nodes visited */
     uint64_t host_tt_hits;  /* host nodes answered by the host table */
+    uint64_t disk_hits;     /* nodes answered by the disk table (pg_perft_deep_disk) */
+    uint64_t disk_stores;   /* nodes written to it */
 } pg_deep_stats;
 int pg_perft_deep(pg_ctx *ctx, pg_tt *tt, const pg_pos48 *pos, uint32_t depth, uint32_t launch_depth,
                   uint32_t divided_depth, int slider, pg_divide_cb cb, void *user, uint64_t out[2],
                   pg_deep_stats *stats);
+/* checkAndCreateDiskHash, diskTTProbe, diskTTStore (launcher.h:1698-1722,
+ * :700-789): pg_perft_deep with a disk table.  hash_path is created with
+ * 2^12 blank 32-byte DiskHashEntry slots (chess.h:518-525) when it does not
+ * exist; nodes of remaining depth disk_depth (0: depth - 2, DISK_HASH_LEVEL
+ * launcher.h:40) are looked up there first, and each one computed is written
+ * and synced (totals below 2^64 only), so a run stopped part way resumes
+ * from the nodes it finished.  One process at a time per file. */
+int pg_perft_deep_disk(pg_ctx *ctx, pg_tt *tt, const pg_pos48 *pos, uint32_t depth, uint32_t launch_depth,
+                       uint32_t divided_depth, int slider, pg_divide_cb cb, void *user, const char *hash_path,
+                       uint32_t disk_depth, uint64_t out[2], pg_deep_stats *stats);
 /* processPerftRecords (launcher.h:1578-1695): perft(depth) of every FEN line
  * of in_path, whose last number is an occurrence count; appends
  * "<line> <perft> <perft * count>" per line to <in_path>.op.  Lines the .op
@@ -348,7 +366,10 @@ int pg_serve_policy(const uint32_t waiting[16], const double oldest[16], uint32_
  *
  * pg_device_issue_probe: one launch of an unrolled stream of one VALU
  * instruction class -- kind 0 v_bitop3_b32, 1 v_xor_b32 (two-operand
- * encoding), 2 v_lshlrev_b64, 3 v_bcnt_u32_b32 -- with eight waves per SIMD;
+ * encoding), 2 v_lshlrev_b64, 3 v_bcnt_u32_b32, 4 v_and_b32, 5 v_or3_b32,
+ * 6 v_lshlrev_b32, 7 v_alignbit_b32, 8 v_add_u32, 9 v_lshl_add_u64,
+ * 10 v_bfrev_b32, 11 v_ffbl_b32, 12 v_mov_b32, 13 v_lshl_or_b32,
+ * 14 v_and_or_b32, 15 v_not_b32 -- with eight waves per SIMD;
  * returns its time, the wave instructions it issued, the engine clock (kHz)
  * and the CU count, from which the issue rate per CU per clock follows. */
 int pg_device_copy(pg_ctx *ctx, uint64_t bytes, uint32_t iters, double *ms);

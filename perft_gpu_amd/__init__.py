@@ -77,7 +77,7 @@ class Stats(ctypes.Structure):
 
 class DeepStats(ctypes.Structure):
     _fields_ = [("total_ms", ctypes.c_double), ("device_ms", ctypes.c_double), ("device_calls", u64),
-                ("host_nodes", u64), ("host_tt_hits", u64)]
+                ("host_nodes", u64), ("host_tt_hits", u64), ("disk_hits", u64), ("disk_stores", u64)]
 
     def as_dict(self):
         return {n: getattr(self, n) for n, _ in self._fields_}
@@ -116,6 +116,7 @@ def lib():
         "pg_host_make_move": (ctypes.c_int, [P(Pos48), ctypes.c_uint16, P(Key128)]),
         "pg_host_children": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_uint32, ctypes.c_void_p, u64, ctypes.c_void_p]),
         "pg_host_count_moves": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_uint32, ctypes.c_int, ctypes.c_void_p]),
+        "pg_cpu_perft": (ctypes.c_int, [P(Pos48), ctypes.c_uint32, P(u64)]),
         "pg_host_slider_attacks": (ctypes.c_int, [ctypes.c_int, ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p,
                                                   ctypes.c_uint32, ctypes.c_void_p]),
         "pg_random_positions": (ctypes.c_int, [u64, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint32,
@@ -156,6 +157,9 @@ def lib():
         "pg_perft_deep": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, P(Pos48), ctypes.c_uint32, ctypes.c_uint32,
                                          ctypes.c_uint32, ctypes.c_int, DIVIDE_CB, ctypes.c_void_p, P(u64),
                                          P(DeepStats)]),
+        "pg_perft_deep_disk": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, P(Pos48), ctypes.c_uint32,
+                                              ctypes.c_uint32, ctypes.c_uint32, ctypes.c_int, DIVIDE_CB, ctypes.c_void_p,
+                                              ctypes.c_char_p, ctypes.c_uint32, P(u64), P(DeepStats)]),
         "pg_perft_records": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_char_p, ctypes.c_uint32,
                                             ctypes.c_int, ctypes.c_uint32, P(ctypes.c_uint32), P(ctypes.c_uint32)]),
         "pg_perft_batch": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_uint32, ctypes.c_uint32,
@@ -218,6 +222,16 @@ def count_moves_host(positions, slider):
     out = np.zeros(len(arr), dtype=np.uint32)
     _check(lib().pg_host_count_moves(_ptr(arr), len(arr), slider, _ptr(out)), "pg_host_count_moves")
     return out
+
+
+def cpu_perft(pos, depth):
+    """perft_bb (perft_bb.h:79-101) on one host thread (pg_cpu_perft): the
+    reference's CPU path as an entry point of its own."""
+    if isinstance(pos, str):
+        pos = read_fen_string(pos)
+    out = u64()
+    _check(lib().pg_cpu_perft(ctypes.byref(pos), depth, ctypes.byref(out)), "pg_cpu_perft")
+    return int(out.value)
 
 
 def children_host(positions, plies=1):
@@ -406,7 +420,10 @@ class Device:
         _check(lib().pg_device_copy(self._h, nbytes, iters, ctypes.byref(ms)), "pg_device_copy")
         return ms.value, 2 * nbytes / (ms.value * 1e-3) / 1e9
 
-    ISSUE_KINDS = {0: "v_bitop3_b32", 1: "v_xor_b32 (two-operand)", 2: "v_lshlrev_b64", 3: "v_bcnt_u32_b32"}
+    ISSUE_KINDS = {0: "v_bitop3_b32", 1: "v_xor_b32 (two-operand)", 2: "v_lshlrev_b64", 3: "v_bcnt_u32_b32",
+                   4: "v_and_b32", 5: "v_or3_b32", 6: "v_lshlrev_b32", 7: "v_alignbit_b32", 8: "v_add_u32",
+                   9: "v_lshl_add_u64", 10: "v_bfrev_b32", 11: "v_ffbl_b32", 12: "v_mov_b32", 13: "v_lshl_or_b32",
+                   14: "v_and_or_b32", 15: "v_not_b32"}
 
     def issue_rate(self, kind, iters=4096):
         """VALU issue probe: (wave instructions per CU per clock, ms, instructions, clock kHz, CUs)."""
@@ -496,13 +513,16 @@ def divide(dev, pos, depth, slider=SLIDER_AUTO, tt=None):
     return {m: int(c) for m, c in zip(moves, counts)}
 
 
-def perft_deep(dev, pos, depth, launch_depth=10, divided_depth=10, tt=None, slider=SLIDER_AUTO, on_divide=None):
+def perft_deep(dev, pos, depth, launch_depth=10, divided_depth=10, tt=None, slider=SLIDER_AUTO, on_divide=None,
+               hash_path=None, disk_depth=0):
     """perft_bb_cpu_launcher + dividedPerft (launcher.h:1181-1450): perft of any
     depth with a 128-bit total.  Nodes deeper than launch_depth are walked on
     the host (with a host table of their totals), each node at launch_depth is
     one device call (TT perft if tt is given).  on_divide(path, count) is
     called for every move of a host node deeper than divided_depth, in the
-    reference's print order.  Returns (total, stats)."""
+    reference's print order.  hash_path: a perfthash.dat disk table for the
+    nodes of remaining depth disk_depth (0: depth - 2), read before and
+    written after each (pg_perft_deep_disk).  Returns (total, stats)."""
     if isinstance(pos, str):
         pos = read_fen_string(pos)
     out = (u64 * 2)()
@@ -512,8 +532,10 @@ def perft_deep(dev, pos, depth, launch_depth=10, divided_depth=10, tt=None, slid
         def _cb(_user, path, lo, hi):
             on_divide(path.decode(), int(lo) | (int(hi) << 64))
         cb = DIVIDE_CB(_cb)
-    _check(lib().pg_perft_deep(dev._h, tt._h if tt is not None else None, ctypes.byref(pos), depth, launch_depth,
-                               divided_depth, slider, cb, None, out, ctypes.byref(st)), "pg_perft_deep")
+    _check(lib().pg_perft_deep_disk(dev._h, tt._h if tt is not None else None, ctypes.byref(pos), depth, launch_depth,
+                                    divided_depth, slider, cb, None,
+                                    os.fsencode(hash_path) if hash_path is not None else None, disk_depth, out,
+                                    ctypes.byref(st)), "pg_perft_deep_disk")
     return int(out[0]) | (int(out[1]) << 64), st.as_dict()
 
 

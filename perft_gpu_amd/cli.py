@@ -16,6 +16,11 @@ with "Perft(NN): <total>, time: .. s, gpuTime: .. s" (launcher.h:1439).
 argv[3] is the GPU count (the reference's usage text calls it launchdepth,
 perft.cu:82); only one GPU is used here.
 
+--disk-hash FILE keeps the totals of the nodes two plies below the root
+in a perfthash.dat-layout file (checkAndCreateDiskHash, launcher.h:1698;
+DISK_HASH_LEVEL 2 for depths of DISK_HASH_MIN_DEPTH 11 and up, :38-40), so
+a deep run that stops part way resumes from the subtrees it finished.
+
 --records runs the reference's records mode (processPerftRecords,
 launcher.h:1578-1695): perft of every FEN line of <file> (its last number
 is an occurrence count) into <file>.op, resuming after the lines already
@@ -30,6 +35,7 @@ import perft_gpu_amd as pg
 
 DIVIDED_PERFT_DEPTH = 10
 DEFAULT_LAUNCH_DEPTH = 10
+DISK_HASH_MIN_DEPTH = 11
 
 
 def main(argv=None):
@@ -44,6 +50,9 @@ def main(argv=None):
     ap.add_argument("--divide", action="store_true", help="print perft(depth-1) per root move at the last depth")
     ap.add_argument("--tt-file", default=None,
                     help="with --tt: resume the table from this checkpoint if it exists, and save it at the end")
+    ap.add_argument("--disk-hash", default=None, help="with --tt: disk table file for the nodes two plies below the root")
+    ap.add_argument("--disk-min-depth", type=int, default=DISK_HASH_MIN_DEPTH,
+                    help="smallest depth that uses the disk table")
     ap.add_argument("--records", default=None, help="records mode: FEN file (last number of a line = count)")
     ap.add_argument("--records-depth", type=int, default=7)
     args = ap.parse_args(argv)
@@ -69,7 +78,8 @@ def main(argv=None):
         try:
             for d in range(min(args.min_depth, args.depth), args.depth + 1):
                 if tt:
-                    divided_perft(dev, tt, pos, d, launch, args.slider)
+                    disk = args.disk_hash if args.disk_hash and d >= args.disk_min_depth else None
+                    divided_perft(dev, tt, pos, d, launch, args.slider, disk)
                     continue
                 t0 = time.perf_counter()
                 res = dev.perft(pos, d, args.slider)
@@ -90,18 +100,21 @@ def main(argv=None):
     return 0
 
 
-def divided_perft(dev, tt, pos, depth, launch, slider):
+def divided_perft(dev, tt, pos, depth, launch, slider, disk=None):
     """dividedPerft (launcher.h:1392-1450): the divided lines of every host
-    node deeper than DIVIDED_PERFT_DEPTH, then the total with wall and GPU time."""
+    node deeper than DIVIDED_PERFT_DEPTH, then the total with wall and GPU
+    time.  disk: a disk table for the nodes of remaining depth depth - 2."""
     def line(path, count):
         print("%s   %20s" % (path, count))
         sys.stdout.flush()
 
     print()
     t0 = time.perf_counter()
-    total, st = pg.perft_deep(dev, pos, depth, min(launch, 15), DIVIDED_PERFT_DEPTH, tt, slider, line)
+    total, st = pg.perft_deep(dev, pos, depth, min(launch, 15), DIVIDED_PERFT_DEPTH, tt, slider, line, disk)
     dt = time.perf_counter() - t0
     print("Perft(%02d):%20s, time: %8g s, gpuTime: %8g s" % (depth, total, dt, st["device_ms"] / 1e3))
+    if disk:
+        print("disk table %s: %d nodes read, %d written" % (disk, st["disk_hits"], st["disk_stores"]))
     sys.stdout.flush()
     return total
 

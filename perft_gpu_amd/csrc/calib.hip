@@ -25,10 +25,17 @@ static __global__ __launch_bounds__(256) void k_copy16(const uint4 *__restrict__
 // instructions per inner step (8 independent chains x 4 rounds)
 constexpr int ISSUE_CHAINS = 8, ISSUE_ROUNDS = 4;
 
-// KIND 0: v_bitop3_b32 (three-input logic, VOP3 encoding)
-// KIND 1: v_xor_b32 (two-operand logic, VOP2 encoding)
-// KIND 2: v_lshlrev_b64 (64-bit shift, VOP3)
-// KIND 3: v_bcnt_u32_b32 (population count, VOP3)
+// KIND  0: v_bitop3_b32 (three-input logic, VOP3)
+//       1: v_xor_b32 (two-operand logic, VOP2)
+//       2: v_lshlrev_b64 (64-bit shift, VOP3)
+//       3: v_bcnt_u32_b32 (population count, VOP3)
+//       4: v_and_b32 (VOP2)            5: v_or3_b32 (VOP3)
+//       6: v_lshlrev_b32 (VOP2)        7: v_alignbit_b32 (VOP3)
+//       8: v_add_u32 (VOP2)            9: v_lshl_add_u64 (64-bit shift-add, VOP3)
+//      10: v_bfrev_b32 (VOP1)         11: v_ffbl_b32 (find first set, VOP1)
+//      12: v_mov_b32 (VOP1)           13: v_lshl_or_b32 (VOP3)
+//      14: v_and_or_b32 (VOP3)        15: v_not_b32 (VOP1)
+constexpr int ISSUE_KINDS = 16;
 template <int KIND>
 __global__ __launch_bounds__(256) void k_issue(u32 *out, u32 iters, u32 seed) {
     u32 a[ISSUE_CHAINS];
@@ -43,14 +50,39 @@ __global__ __launch_bounds__(256) void k_issue(u32 *out, u32 iters, u32 seed) {
         for (int r = 0; r < ISSUE_ROUNDS; r++) {
 #pragma unroll
             for (int j = 0; j < ISSUE_CHAINS; j++) {
+                const u32 b = a[(j + 1) % ISSUE_CHAINS], c = a[(j + 3) % ISSUE_CHAINS];
                 if constexpr (KIND == 0) {
-                    a[j] = __builtin_amdgcn_bitop3_b32(a[j], a[(j + 1) % ISSUE_CHAINS], a[(j + 3) % ISSUE_CHAINS], 0x96);
+                    a[j] = __builtin_amdgcn_bitop3_b32(a[j], b, c, 0x96);
                 } else if constexpr (KIND == 1) {
-                    asm volatile("v_xor_b32_e32 %0, %1, %0" : "+v"(a[j]) : "v"(a[(j + 1) % ISSUE_CHAINS]));
+                    asm volatile("v_xor_b32_e32 %0, %1, %0" : "+v"(a[j]) : "v"(b));
                 } else if constexpr (KIND == 2) {
                     asm volatile("v_lshlrev_b64 %0, 1, %0" : "+v"(w[j]));
+                } else if constexpr (KIND == 3) {
+                    asm volatile("v_bcnt_u32_b32 %0, %1, %0" : "+v"(a[j]) : "v"(b));
+                } else if constexpr (KIND == 4) {
+                    asm volatile("v_and_b32_e32 %0, %1, %0" : "+v"(a[j]) : "v"(b));
+                } else if constexpr (KIND == 5) {
+                    asm volatile("v_or3_b32 %0, %0, %1, %2" : "+v"(a[j]) : "v"(b), "v"(c));
+                } else if constexpr (KIND == 6) {
+                    asm volatile("v_lshlrev_b32_e32 %0, 1, %0" : "+v"(a[j]));
+                } else if constexpr (KIND == 7) {
+                    asm volatile("v_alignbit_b32 %0, %0, %1, 7" : "+v"(a[j]) : "v"(b));
+                } else if constexpr (KIND == 8) {
+                    asm volatile("v_add_u32_e32 %0, %1, %0" : "+v"(a[j]) : "v"(b));
+                } else if constexpr (KIND == 9) {
+                    asm volatile("v_lshl_add_u64 %0, %0, 1, %1" : "+v"(w[j]) : "v"(w[(j + 1) % ISSUE_CHAINS]));
+                } else if constexpr (KIND == 10) {
+                    asm volatile("v_bfrev_b32_e32 %0, %0" : "+v"(a[j]));
+                } else if constexpr (KIND == 11) {
+                    asm volatile("v_ffbl_b32_e32 %0, %0" : "+v"(a[j]));
+                } else if constexpr (KIND == 12) {
+                    asm volatile("v_mov_b32_e32 %0, %1" : "=v"(a[j]) : "v"(b));
+                } else if constexpr (KIND == 13) {
+                    asm volatile("v_lshl_or_b32 %0, %0, 3, %1" : "+v"(a[j]) : "v"(b));
+                } else if constexpr (KIND == 14) {
+                    asm volatile("v_and_or_b32 %0, %0, %1, %2" : "+v"(a[j]) : "v"(b), "v"(c));
                 } else {
-                    asm volatile("v_bcnt_u32_b32 %0, %1, %0" : "+v"(a[j]) : "v"(a[(j + 1) % ISSUE_CHAINS]));
+                    asm volatile("v_not_b32_e32 %0, %0" : "+v"(a[j]));
                 }
             }
         }
@@ -59,6 +91,12 @@ __global__ __launch_bounds__(256) void k_issue(u32 *out, u32 iters, u32 seed) {
 #pragma unroll
     for (int j = 0; j < ISSUE_CHAINS; j++) x ^= a[j] ^ (u32)w[j] ^ (u32)(w[j] >> 32);
     if (x == 0x12345678u) out[blockIdx.x] = x;  // keeps the chains live
+}
+
+template <int K>
+static void launch_issue(int kind, u32 blocks, hipStream_t s, u32 *out, u32 iters) {
+    if (kind == K) k_issue<K><<<blocks, 256, 0, s>>>(out, iters, 7);
+    else if constexpr (K + 1 < ISSUE_KINDS) launch_issue<K + 1>(kind, blocks, s, out, iters);
 }
 
 }  // namespace pg
@@ -93,20 +131,13 @@ int pg_device_copy(pg_ctx *ctx, uint64_t bytes, uint32_t iters, double *ms_out) 
 int pg_device_issue_probe(pg_ctx *ctx, int kind, uint32_t iters, double *ms_out, uint64_t *wave_insts,
                           uint32_t *clock_khz, uint32_t *cus) {
     Engine *c = reinterpret_cast<Engine *>(ctx);
-    if (!c || kind < 0 || kind > 3 || iters == 0 || iters > (1u << 24)) return PG_EINVAL;
+    if (!c || kind < 0 || kind >= ISSUE_KINDS || iters == 0 || iters > (1u << 24)) return PG_EINVAL;
     (void)hipSetDevice(c->device);
     c->top = 0;
     const u32 blocks = (u32)c->cus * 8;  // 32 waves per CU, eight per SIMD
     u32 *out = (u32 *)c->alloc(blocks * sizeof(u32));
     if (!out) return PG_ENOMEM;
-    auto launch = [&](u32 it) {
-        switch (kind) {
-        case 0: k_issue<0><<<blocks, 256, 0, c->stream>>>(out, it, 7); break;
-        case 1: k_issue<1><<<blocks, 256, 0, c->stream>>>(out, it, 7); break;
-        case 2: k_issue<2><<<blocks, 256, 0, c->stream>>>(out, it, 7); break;
-        default: k_issue<3><<<blocks, 256, 0, c->stream>>>(out, it, 7); break;
-        }
-    };
+    auto launch = [&](u32 it) { launch_issue<0>(kind, blocks, c->stream, out, it); };
     launch(iters);  // warm-up
     if (hipEventRecord(c->ev0, c->stream) != hipSuccess) return PG_EHIP;
     launch(iters);

@@ -1,6 +1,8 @@
 // capi_host.cpp -- host-side entry points of the C ABI: FEN conversion,
-// Zobrist keys, host move listing and the seeded position generator.
-// None of these computes a perft count; perft runs only on the device.
+// Zobrist keys, host move listing, the seeded position generator, and
+// pg_cpu_perft, the reference's single-thread CPU perft as its own entry
+// point (no device entry point calls it: device perft never falls back to
+// the host).
 #include <cstdio>
 #include <cstring>
 #include <algorithm>
@@ -8,6 +10,10 @@
 #include "tables.h"
 
 using namespace pg;
+
+namespace pg {
+void set_error(const char *fmt, ...);  // engine.hip
+}
 
 namespace {
 
@@ -178,6 +184,47 @@ int pg_host_count_moves(const pg_pos48 *pos, uint32_t n, int slider, uint32_t *o
         default: out[i] = count_any<SL_BYTE>(p, t); break;
         }
     }
+    return PG_OK;
+}
+
+}  // extern "C"
+
+namespace {
+// perft_bb (perft_bb.h:79-101): children by generation and make, the
+// legal-move count at depth 1
+u64 host_perft(const Pos &p, u32 depth, const Tables &t, bool *ovf) {
+    if (depth == 0) return 1;
+    if (depth == 1) return count_any<SL_FANCY>(p, t);
+    u16 mv[256];
+    const u32 n = generate_any<SL_FANCY>(p, t, mv, 256);
+    u64 s = 0;
+    for (u32 i = 0; i < n && !*ovf; i++) {
+        Pos q = p;
+        make_any(q, mv[i], t);
+        const u64 c = host_perft(q, depth - 1, t, ovf);
+        if (s + c < s) *ovf = true;
+        s += c;
+    }
+    return s;
+}
+}  // namespace
+
+extern "C" {
+
+int pg_cpu_perft(const pg_pos48 *pos, uint32_t depth, uint64_t *out) {
+    const HostTables *h = tables_or_null();
+    if (!h) return PG_ETABLE;
+    if (!pos || !out || depth > 15) return PG_EINVAL;
+    Pos p;
+    memcpy(&p, pos, sizeof(p));
+    if (!pos_sane(p)) return PG_EINVAL;
+    bool ovf = false;
+    const u64 v = host_perft(p, depth, h->view(), &ovf);
+    if (ovf) {
+        set_error("perft(%u) does not fit 64 bits", depth);
+        return PG_EOVERFLOW;
+    }
+    *out = v;
     return PG_OK;
 }
 

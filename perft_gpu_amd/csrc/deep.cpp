@@ -9,6 +9,20 @@
 // by (128-bit Zobrist key, remaining depth); every node at the launch depth is
 // one device call (pg_perft_tt with a table, pg_perft without), whose 64-bit
 // result is overflow-checked on the device.
+//
+// With a disk table (pg_perft_deep_disk) the nodes at one remaining depth are
+// also looked up in, and written to, a file laid out as the reference's
+// perfthash.dat (checkAndCreateDiskHash, launcher.h:1698-1722; DiskHashEntry,
+// chess.h:518-525): 2^12 index entries of 32 bytes, then chained entries
+// appended at the end; an entry is {key lo, key hi, perft, depth, next}, an
+// all-zero key is a free slot, next = ~0 ends a chain.  Every finished node
+// at that depth is written (and synced) before the walk goes on, so a run
+// that is interrupted resumes from the subtrees it has finished.  In the
+// reference the probe and store are compiled out (diskTTProbe/diskTTStore,
+// launcher.h:700-789, used under ENABLE_DISK_HASH == 7 at :1220-1231 and
+// :1381-1386); the file is created and left blank.
+#include <fcntl.h>
+#include <unistd.h>
 #include <algorithm>
 #include <chrono>
 #include <cstdio>
@@ -46,6 +60,83 @@ void move_string(const pg_pos48 &p, pg_move m, char out[8]) {
     snprintf(out, 8, "%c%d%c%c%d ", 'a' + (from & 7), (from >> 3) + 1, cap ? '*' : '-', 'a' + (to & 7), (to >> 3) + 1);
 }
 
+// perfthash.dat: 32-byte entries, 2^12 of them indexed by the low key bits
+struct DiskEntry {
+    uint64_t lo, hi, perft;
+    uint32_t depth, next;
+};
+static_assert(sizeof(DiskEntry) == 32, "DiskHashEntry is 32 bytes");
+constexpr uint64_t DISK_INDEX = 1u << 12;
+constexpr uint32_t DISK_END = ~0u;
+
+struct DiskTable {
+    int fd = -1;
+    uint64_t entries = 0;  // file length in entries
+
+    int open_file(const char *path) {
+        fd = open(path, O_RDWR | O_CREAT, 0644);
+        if (fd < 0) return PG_EINVAL;
+        const off_t len = lseek(fd, 0, SEEK_END);
+        if (len < 0 || len % (off_t)sizeof(DiskEntry)) return PG_EINVAL;
+        entries = (uint64_t)len / sizeof(DiskEntry);
+        if (entries == 0) {  // a new table: the blank index part
+            std::vector<DiskEntry> blank(DISK_INDEX);
+            memset(blank.data(), 0, blank.size() * sizeof(DiskEntry));
+            if (pwrite(fd, blank.data(), blank.size() * sizeof(DiskEntry), 0) != (ssize_t)(blank.size() * sizeof(DiskEntry)))
+                return PG_EINVAL;
+            entries = DISK_INDEX;
+        }
+        return entries >= DISK_INDEX ? PG_OK : PG_EINVAL;
+    }
+    ~DiskTable() {
+        if (fd >= 0) close(fd);
+    }
+    bool read_at(uint64_t i, DiskEntry &e) const {
+        return i < entries && pread(fd, &e, sizeof(e), (off_t)(i * sizeof(e))) == (ssize_t)sizeof(e);
+    }
+    bool write_at(uint64_t i, const DiskEntry &e) const {
+        return pwrite(fd, &e, sizeof(e), (off_t)(i * sizeof(e))) == (ssize_t)sizeof(e);
+    }
+    // 1: found (*v), 0: absent, < 0: error
+    int probe(uint64_t lo, uint64_t hi, uint32_t depth, uint64_t *v) const {
+        uint64_t i = lo & (DISK_INDEX - 1);
+        for (uint64_t steps = 0; steps <= entries; steps++) {
+            DiskEntry e;
+            if (!read_at(i, e)) return PG_EINVAL;
+            if (e.lo == 0 && e.hi == 0) return 0;
+            if (e.lo == lo && e.hi == hi && e.depth == depth) {
+                *v = e.perft;
+                return 1;
+            }
+            if (e.next == DISK_END) return 0;
+            i = e.next;
+        }
+        return PG_EINVAL;  // a chain that loops: not a table this code wrote
+    }
+    // into the first free slot of the key's chain, or a new entry at the end
+    int store(uint64_t lo, uint64_t hi, uint32_t depth, uint64_t v) {
+        uint64_t i = lo & (DISK_INDEX - 1);
+        const DiskEntry mine{lo, hi, v, depth, DISK_END};
+        for (uint64_t steps = 0; steps <= entries; steps++) {
+            DiskEntry e;
+            if (!read_at(i, e)) return PG_EINVAL;
+            if (e.lo == 0 && e.hi == 0) return write_at(i, mine) && fdatasync(fd) == 0 ? PG_OK : PG_EINVAL;
+            if (e.next == DISK_END) {
+                const uint64_t j = entries;
+                if (j >= DISK_END) return PG_ENOMEM;
+                // the new entry first, then the link to it, so an interrupted
+                // store leaves at worst an unreachable entry
+                if (!write_at(j, mine)) return PG_EINVAL;
+                entries++;
+                e.next = (uint32_t)j;
+                return write_at(i, e) && fdatasync(fd) == 0 ? PG_OK : PG_EINVAL;
+            }
+            i = e.next;
+        }
+        return PG_EINVAL;
+    }
+};
+
 struct DeepWalk {
     pg_ctx *ctx;
     pg_tt *tt;
@@ -57,6 +148,8 @@ struct DeepWalk {
     size_t maxEntries = (size_t)1 << 22;
     pg_deep_stats st{};
     int rc = PG_OK;
+    DiskTable *disk = nullptr;
+    uint32_t diskDepth = 0;
 
     u128 device(const pg_pos48 &p, uint32_t depth) {
         uint64_t v = 0;
@@ -70,6 +163,28 @@ struct DeepWalk {
 
     u128 walk(const pg_pos48 &p, const pg_key128 &key, uint32_t depth, std::string &path) {
         if (rc != PG_OK) return 0;
+        if (disk && depth == diskDepth) {
+            uint64_t v = 0;
+            const int r = disk->probe(key.lo, key.hi, depth, &v);
+            if (r < 0) {
+                rc = r;
+                return 0;
+            }
+            if (r == 1) {
+                st.disk_hits++;
+                return v;
+            }
+            const u128 total = walk_node(p, key, depth, path);
+            // totals of 2^64 and above are not kept (the entry holds 64 bits)
+            if (rc == PG_OK && (total >> 64) == 0) {
+                if ((rc = disk->store(key.lo, key.hi, depth, (uint64_t)total)) == PG_OK) st.disk_stores++;
+            }
+            return total;
+        }
+        return walk_node(p, key, depth, path);
+    }
+
+    u128 walk_node(const pg_pos48 &p, const pg_key128 &key, uint32_t depth, std::string &path) {
         if (depth <= launch) return device(p, depth);
         st.host_nodes++;
         const NodeKey nk{key.lo, key.hi, depth};
@@ -104,12 +219,20 @@ struct DeepWalk {
 
 }  // namespace
 
-extern "C" int pg_perft_deep(pg_ctx *ctx, pg_tt *tt, const pg_pos48 *pos, uint32_t depth, uint32_t launch_depth,
-                             uint32_t divided_depth, int slider, pg_divide_cb cb, void *user, uint64_t out[2],
-                             pg_deep_stats *stats) {
+extern "C" int pg_perft_deep_disk(pg_ctx *ctx, pg_tt *tt, const pg_pos48 *pos, uint32_t depth, uint32_t launch_depth,
+                                  uint32_t divided_depth, int slider, pg_divide_cb cb, void *user,
+                                  const char *hash_path, uint32_t disk_depth, uint64_t out[2], pg_deep_stats *stats) {
     if (!ctx || !pos || !out || depth > 64 || launch_depth < 1 || launch_depth > 15) return PG_EINVAL;
     const double t0 = now_ms();
     DeepWalk w{ctx, tt, slider, launch_depth, divided_depth, cb, user, {}, (size_t)1 << 22, {}, PG_OK};
+    DiskTable disk;
+    if (hash_path) {
+        // DISK_HASH_LEVEL (launcher.h:40, :1463): two plies below the root
+        if (disk_depth == 0) disk_depth = depth > 2 ? depth - 2 : 1;
+        if (disk_depth > depth || disk.open_file(hash_path) != PG_OK) return PG_EINVAL;
+        w.disk = &disk;
+        w.diskDepth = disk_depth;
+    }
     pg_key128 key;
     int rc = pg_zobrist128(pos, &key);
     if (rc != PG_OK) return rc;
@@ -121,4 +244,11 @@ extern "C" int pg_perft_deep(pg_ctx *ctx, pg_tt *tt, const pg_pos48 *pos, uint32
     out[0] = (uint64_t)total;
     out[1] = (uint64_t)(total >> 64);
     return PG_OK;
+}
+
+extern "C" int pg_perft_deep(pg_ctx *ctx, pg_tt *tt, const pg_pos48 *pos, uint32_t depth, uint32_t launch_depth,
+                             uint32_t divided_depth, int slider, pg_divide_cb cb, void *user, uint64_t out[2],
+                             pg_deep_stats *stats) {
+    return pg_perft_deep_disk(ctx, tt, pos, depth, launch_depth, divided_depth, slider, cb, user, nullptr, 0, out,
+                              stats);
 }

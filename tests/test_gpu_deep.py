@@ -73,6 +73,34 @@ def test_totals_past_2_pow_64_are_exact(dev, pg):
     assert total == want + 20 * (1 << 63) and total >= 1 << 64
 
 
+def test_disk_table_resume(dev, pg, tmp_path):
+    """The disk table (perfthash.dat layout) of the nodes two plies below the
+    root: a table that holds one root move's subtree -- a run stopped after
+    that move -- answers those nodes and the rest are computed and written; a
+    full table answers every node with no device call.  Each total is the
+    published value."""
+    path = str(tmp_path / "perfthash.dat")
+    want = W.CPW_COUNTS["kiwipete"][6]
+    root = pg.read_fen_string(W.CPW["kiwipete"])
+    child, _ = pg.make_move_host(root, pg.generate_moves_host(root)[0])
+    first = {pg.zobrist128(pg.make_move_host(child, m)[0]) for m in pg.generate_moves_host(child)}
+    ply2 = {pg.zobrist128(pg.make_move_host(pg.make_move_host(root, a)[0], b)[0])
+            for a in pg.generate_moves_host(root) for b in pg.generate_moves_host(pg.make_move_host(root, a)[0])}
+    sub, st = pg.perft_deep(dev, child, 5, 4, 10, hash_path=path, disk_depth=4)
+    assert sub == dev.perft(child, 5, 0)
+    assert (st["disk_stores"], st["disk_hits"]) == (len(first), 0)
+    assert os.path.getsize(path) >= 4096 * 32 and os.path.getsize(path) % 32 == 0
+    print("kiwipete d6 with the partial table", flush=True)
+    total, st = pg.perft_deep(dev, W.CPW["kiwipete"], 6, 4, 10, hash_path=path, disk_depth=4)
+    assert total == want
+    # 2039 ply-2 nodes; a position met again (a transposition) is read back
+    # from the table too
+    assert st["disk_stores"] == len(ply2 - first) and st["disk_hits"] == 2039 - len(ply2 - first)
+    total, st = pg.perft_deep(dev, W.CPW["kiwipete"], 6, 4, 10, hash_path=path, disk_depth=4)
+    assert total == want
+    assert (st["disk_hits"], st["disk_stores"], st["device_calls"]) == (2039, 0, 0)
+
+
 def test_cli_divided_output(pg, capsys):
     from perft_gpu_amd import cli
     cli.main([W.CPW["pos3"], "5", "1", "3", "--tt", "1", "--min-depth", "4"])

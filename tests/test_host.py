@@ -212,3 +212,17 @@ def test_host_count_special_rule_trees(pg, orc, v):
     got = pg.count_moves_host(arr, v)
     want = np.array([orc.count_moves(b) for b in boards], np.uint32)
     assert np.array_equal(got, want), np.nonzero(got != want)[0][:5]
+
+
+def test_cpu_perft_entry_point(pg):
+    """pg_cpu_perft (perft_bb, perft_bb.h:79-101, one host thread) against the
+    published counts and the reference's own outputs for random and edge
+    positions (fixtures written by the reference compiled here)."""
+    for name, depth in (("start", 4), ("kiwipete", 3), ("pos3", 4), ("pos4", 3), ("pos4_mirror", 3)):
+        assert pg.cpu_perft(W.CPW[name], depth) == W.CPW_COUNTS[name][depth]
+    for item in golden("random.json")[:64] + golden("edge.json"):
+        p = pg.pos_from_words(item["words"])
+        for d, want in enumerate(item["perft"][:4]):
+            assert pg.cpu_perft(p, d) == want, (item["fen"], d)
+    with pytest.raises(pg.PerftError):
+        pg.cpu_perft(W.CPW["start"], 16)
