@@ -7,9 +7,11 @@
 //    ~own, single check -> checker | squares between, double check -> 0.
 //    A pinned piece can never land on a check-resolving square, so the
 //    result equals the reference's countMovesOutOfCheck.
-//  * Pawns and knights are counted in bulk (one shift + popcount per
-//    direction).  Pinned pawns move only along the king line they sit on,
-//    which is applied as a per-direction mask instead of a per-pawn loop.
+//  * Pawns are counted in bulk (one shift + popcount per direction); pinned
+//    pawns move only along the king line they sit on, which is applied as a
+//    per-direction mask instead of a per-pawn loop.  Knights take one table
+//    lookup + popcount each (at most two knights is cheaper than eight
+//    direction shifts).
 //  * With Kogge-Stone sliders, moves are also counted in bulk: the rays of
 //    different sliders in one direction are disjoint (a ray stops at the
 //    first occupied square), so popcount(fill_d(all movers) & target) sums
