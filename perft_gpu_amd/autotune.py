@@ -15,8 +15,8 @@ positions, every combination of
                  batch's positions: -1 sized to the batch's largest level, 0
                  the whole chip (pg_set_batch_grid)
   chunk          positions per graph launch (None: the whole batch); the
-                 default space holds only None -- one launch always won on
-                 the batches measured -- and callers may pass candidates
+                 default space holds only None (one launch per batch), and
+                 callers may pass candidates
 and returns the fastest.  Every timed run is checked against the first
 configuration's counts, so a configuration that changed a result could never
 win.  The reference has no autotuner; its choices are compile-time switches
