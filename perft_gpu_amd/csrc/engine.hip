@@ -126,7 +126,7 @@ template <int V> static void launch_leaf_fused(Engine &c, const Pos *b, const u3
         (void)c; (void)b; (void)cnt; (void)n;
     } else {
         k_leaf_fused<V, false><<<c.leaf_grid(n), LEAF_WG, 0, c.stream>>>(b, cnt, n, c.dAcc, c.dItems, nullptr, c.dErr,
-                                                                         c.dOvf, c.dt);
+                                                                         c.dOvf, c.dt, nullptr);
     }
 }
 

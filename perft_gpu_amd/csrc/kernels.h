@@ -298,10 +298,12 @@ enum : u8 { ST_LIVE = 0, ST_HIT = 1, ST_DUP = 2 };
 // The fused leaf over boards [0, n) with move counts cnt[]: PER = false adds
 // every count to *acc; PER = true writes base + perft(2) to per[i] for live
 // boards (state[i] == ST_LIVE when state is given, else a valid board).
+// With idx, the boards are idx[0..n) (live boards with moves, listed by
+// k_live_write) and per[] is written at those indices.
 template <int V, bool PER>
 __device__ __forceinline__ void leaf_tiles(const Pos *__restrict__ boards, const u32 *__restrict__ cnt, u64 n, u64 *acc,
                                            u64 *items, u64 *__restrict__ per, const u8 *__restrict__ state, u64 base,
-                                           u32 *err, u32 *ovf, const Tables &t) {
+                                           u32 *err, u32 *ovf, const Tables &t, const u32 *__restrict__ idx = nullptr) {
     static_assert(V != SL_BYTE, "the byte-lookup tables leave no LDS for the move window");
     __shared__ Pos sb[LEAF_WG];
     constexpr int LEAF_CAP = leaf_cap<PER>();
@@ -313,8 +315,9 @@ __device__ __forceinline__ void leaf_tiles(const Pos *__restrict__ boards, const
     const Tables &tl = t;
     u64 s = 0, itemSum = 0;
     for (u64 tile = blockIdx.x; tile * LEAF_WG < n; tile += gridDim.x) {
-        const u64 i = tile * LEAF_WG + tid;
-        const bool live = i < n;
+        const u64 t0 = tile * LEAF_WG + tid;
+        const bool live = t0 < n;
+        const u64 i = idx && live ? (u64)idx[t0] : t0;
         const u32 c = live ? cnt[i] : 0;
         if (live) sb[tid] = boards[i];
         if (PER) sper[tid] = 0;
@@ -347,7 +350,7 @@ __device__ __forceinline__ void leaf_tiles(const Pos *__restrict__ boards, const
             __syncthreads();
             start = wend;
         }
-        if (PER && live && (state ? state[i] == ST_LIVE : pos_valid(sb[tid]))) {
+        if (PER && live && (idx || (state ? state[i] == ST_LIVE : pos_valid(sb[tid])))) {
             const u64 v = base + sper[tid];
             if (v < base) *ovf = 1;
             per[i] = v;
@@ -364,8 +367,44 @@ __device__ __forceinline__ void leaf_tiles(const Pos *__restrict__ boards, const
 template <int V, bool PER>
 __global__ __launch_bounds__(LEAF_WG) __attribute__((amdgpu_waves_per_eu(V == SL_KOGGE ? 4 : 5, 8))) void k_leaf_fused(
     const Pos *__restrict__ boards, const u32 *__restrict__ cnt, u64 n, u64 *acc, u64 *items, u64 *__restrict__ per,
-    u32 *err, u32 *ovf, Tables t) {
-    leaf_tiles<V, PER>(boards, cnt, n, acc, items, per, nullptr, 0, err, ovf, t);
+    u32 *err, u32 *ovf, Tables t, const u32 *__restrict__ idx) {
+    leaf_tiles<V, PER>(boards, cnt, n, acc, items, per, nullptr, 0, err, ovf, t, idx);
+}
+
+// ---------------------------------------------------------------- live list
+// The boards of a leaf level that have moves, in index order: a level whose
+// other boards were resolved from the table or collapsed into an equal board
+// (count 0) then fills the fused leaf's tiles with boards that have work.
+// Each work-group takes LIVE_CHUNK consecutive boards: k_live_count counts
+// its live boards, the counts are scanned, and k_live_write lists them at
+// the chunk's offset.
+constexpr int LIVE_WG = 256, LIVE_PER_THREAD = 16, LIVE_CHUNK = LIVE_WG * LIVE_PER_THREAD;
+
+static __global__ __launch_bounds__(LIVE_WG) void k_live_count(const u32 *__restrict__ cnt, u64 n, u32 *__restrict__ chunkCount) {
+    const u64 base = (u64)blockIdx.x * LIVE_CHUNK;
+    u32 c = 0;
+#pragma unroll
+    for (int k = 0; k < LIVE_PER_THREAD; k++) {
+        const u64 i = base + (u64)k * LIVE_WG + threadIdx.x;
+        c += i < n && cnt[i] != 0;
+    }
+    u64 total;
+    (void)block_excl_scan<LIVE_WG>(c, &total);
+    if (threadIdx.x == 0) chunkCount[blockIdx.x] = (u32)total;
+}
+
+static __global__ __launch_bounds__(LIVE_WG) void k_live_write(const u32 *__restrict__ cnt, u64 n,
+                                                              const u64 *__restrict__ chunkOff, u32 *__restrict__ idx) {
+    const u64 base = (u64)blockIdx.x * LIVE_CHUNK;
+    u64 at = chunkOff[blockIdx.x];
+    for (int k = 0; k < LIVE_PER_THREAD; k++) {
+        const u64 i = base + (u64)k * LIVE_WG + threadIdx.x;
+        const bool f = i < n && cnt[i] != 0;
+        u64 total;
+        const u64 pos = block_excl_scan<LIVE_WG>(f ? 1 : 0, &total);
+        if (f) idx[at + pos] = (u32)i;
+        at += total;
+    }
 }
 
 // ---------------------------------------------------------------- scan

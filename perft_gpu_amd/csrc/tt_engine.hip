@@ -25,10 +25,11 @@ template <int V> static void launch_expand_tt(Engine &c, TTable &tt, const Pos *
     constexpr int W = wg_size<V>();
     k_expand_tt<V><<<c.grid(m, W), W, 0, c.stream>>>(B, K, par, mv, m, C, CK, CP, Ccnt, S, D, tt.view, rc, tt.dCtr, c.dt);
 }
-template <int V> static void launch_leaf_per(Engine &c, const Pos *B, const u32 *cnt, u64 n, u64 *P) {
+template <int V>
+static void launch_leaf_per(Engine &c, const Pos *B, const u32 *cnt, u64 n, u64 *P, const u32 *idx) {
     if constexpr (V != SL_BYTE)
         k_leaf_fused<V, true><<<c.leaf_grid(n), LEAF_WG, 0, c.stream>>>(B, cnt, n, c.dAcc, c.dItems, P, c.dErr, c.dOvf,
-                                                                        c.dt);
+                                                                        c.dt, idx);
 }
 template <int V> static void launch_gen(Engine &c, const Pos *b, u64 n, const u64 *off, u16 *mv, u32 *par) {
     constexpr int W = wg_size<V>();
@@ -66,8 +67,36 @@ int Engine::tt_bfs(int v, TTable &tt, Pos *B, Key128 *K, u64 n, u32 *cnt, int r,
         return PG_OK;
     }
     if (r == 2 && v != SL_BYTE) {
+        // Boards resolved from the table or collapsed into an equal board
+        // have count 0 and already hold their perft (a live board without
+        // moves holds 0), so the leaf runs over the list of boards with
+        // moves: its tiles are then full of work.  Without room for the list
+        // the leaf takes every board and skips the others itself.
+        const size_t mark = top;
+        const u64 chunks = (n + LIVE_CHUNK - 1) / LIVE_CHUNK;
+        u32 *chunkCount = n <= 0xFFFFFFFFULL ? (u32 *)alloc(chunks * sizeof(u32)) : nullptr;
+        u64 *chunkOff = chunkCount ? (u64 *)alloc((chunks + 1) * sizeof(u64)) : nullptr;
+        u32 *idx = nullptr;
+        u64 nLive = n;
+        if (chunkOff) {
+            k_live_count<<<(u32)chunks, LIVE_WG, 0, stream>>>(cnt, n, chunkCount);
+            HIP_TRY(hipGetLastError());
+            int rc = scan(chunkCount, chunks, chunkOff, &nLive);
+            if (rc && rc != PG_ENOMEM) {
+                top = mark;
+                return rc;
+            }
+            idx = rc == PG_OK ? (u32 *)alloc(nLive * sizeof(u32) + 4) : nullptr;
+            if (idx) {
+                k_live_write<<<(u32)chunks, LIVE_WG, 0, stream>>>(cnt, n, chunkOff, idx);
+                HIP_TRY(hipGetLastError());
+            } else {
+                nLive = n;
+            }
+        }
         if (timing) HIP_TRY(hipEventRecord(ev0, stream));
-        PG_VARIANT_CALL(v, launch_leaf_per, *this, B, cnt, n, P);
+        if (nLive) PG_VARIANT_CALL(v, launch_leaf_per, *this, B, cnt, nLive, P, idx);
+        top = mark;
         if (ply == 0 && bias) k_add_base<<<1, 64, 0, stream>>>(P, n, bias, dOvf);
         HIP_TRY(hipGetLastError());
         if (timing) {
