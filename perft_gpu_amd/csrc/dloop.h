@@ -26,6 +26,7 @@ struct DLevel {
     u8 *state;
     u32 *dup;
     u64 *tiles;       // scan tile sums
+    u32 *live;        // leaf level: the boards with moves (kd_live), when there was room
     u16 *mv;          // transient: moves / parents of the children
     u32 *par;
     u32 *set;         // transient: duplicate set
@@ -42,6 +43,7 @@ struct Plan {
     u64 bias;           // test hook: starting value of every root's total
     u32 ovf;            // a 64-bit total wrapped
     u64 peak;           // highest bump top reached (bytes of the region used)
+    u64 liveN;          // entries of the leaf level's live list
     DLevel lv[TT_MAX_DEPTH];
 };
 
@@ -51,6 +53,15 @@ __device__ __forceinline__ void *plan_alloc(Plan *P, u64 bytes) {
         P->overflow = 1;
         return nullptr;
     }
+    P->top = a + bytes;
+    if (P->top > P->peak) P->peak = P->top;
+    return P->base + a;
+}
+
+// as plan_alloc, for an optional array: no room leaves the plan as it was
+__device__ __forceinline__ void *plan_try_alloc(Plan *P, u64 bytes) {
+    const u64 a = (P->top + 255) & ~255ULL;
+    if (a + bytes > P->cap) return nullptr;
     P->top = a + bytes;
     if (P->top > P->peak) P->peak = P->top;
     return P->base + a;
@@ -314,14 +325,59 @@ static __global__ __launch_bounds__(256) void kd_dedup(Plan *P, int l, TTCounter
     }
 }
 
+// The live list of the leaf level l (as k_live_count / k_live_write, in one
+// pass): boards resolved from the table or collapsed into an equal board
+// have count 0, so the fused leaf runs over the boards with moves only.  A
+// live board without moves gets its perft(2) = base here.  kd_plan_live
+// makes room for the list (none: the leaf takes every board); each
+// work-group lists a chunk of boards in order at an offset it reserves with
+// one atomic, so chunks land in any order.
+static __global__ void kd_plan_live(Plan *P, int l) {
+    if (threadIdx.x || blockIdx.x) return;
+    if (l > 0) release_level(P, l - 1);  // the moves, parents and set that made level l
+    DLevel &L = P->lv[l];
+    P->liveN = 0;
+    L.live = L.n && L.n <= 0xFFFFFFFFULL ? (u32 *)plan_try_alloc(P, L.n * 4) : nullptr;
+}
+
+static __global__ __launch_bounds__(LIVE_WG) void kd_live(Plan *P, int l) {
+    const DLevel L = P->lv[l];
+    if (!L.live) return;
+    __shared__ u64 sBase;
+    const u64 base = l == 0 ? P->bias : 0;
+    const u64 chunks = (L.n + LIVE_CHUNK - 1) / LIVE_CHUNK;
+    for (u64 ch = blockIdx.x; ch < chunks; ch += gridDim.x) {
+        // each thread takes LIVE_PER_THREAD consecutive boards, so the list
+        // keeps the level's order (siblings stay neighbours in the leaf)
+        const u64 first = ch * LIVE_CHUNK + (u64)threadIdx.x * LIVE_PER_THREAD;
+        u32 f = 0, c = 0;
+#pragma unroll
+        for (int k = 0; k < LIVE_PER_THREAD; k++) {
+            const u64 i = first + k;
+            const u32 n = i < L.n ? L.cnt[i] : 0;
+            if (n) f |= 1u << k;
+            c += n != 0;
+            if (i < L.n && n == 0 && L.state[i] == ST_LIVE) L.perft[i] = base;
+        }
+        u64 total;
+        const u64 pos = block_excl_scan<LIVE_WG>(c, &total);
+        if (threadIdx.x == 0) sBase = total ? atomicAdd((unsigned long long *)&P->liveN, total) : 0;
+        __syncthreads();
+        u64 at = sBase + pos;
+        for (; f; f &= f - 1) L.live[at++] = (u32)(first + __builtin_ctz(f));
+        __syncthreads();
+    }
+}
+
 // perft(2) of the boards of level l, fused (see leaf_tiles): per board into
 // the level's perft array, or (TOTAL, a single-root search that only needs
 // the sum) added to the plan's total
 template <int V, bool TOTAL>
 __global__ __launch_bounds__(LEAF_WG) __attribute__((amdgpu_waves_per_eu(V == SL_KOGGE ? 4 : 5, 8))) void kd_leaf(Plan *P, int l, Tables t) {
     const DLevel L = P->lv[l];
-    leaf_tiles<V, !TOTAL>(L.b, L.cnt, L.n, &P->acc, &P->leafItems, L.perft, L.state, l == 0 ? P->bias : 0, &P->err,
-                          &P->ovf, t);
+    const bool listed = !TOTAL && L.live;
+    leaf_tiles<V, !TOTAL>(L.b, L.cnt, listed ? P->liveN : L.n, &P->acc, &P->leafItems, L.perft, L.state,
+                          l == 0 ? P->bias : 0, &P->err, &P->ovf, t, listed ? L.live : nullptr);
 }
 
 // the plan's total += the move counts of level l (perft(1) of a total-only search)

@@ -159,6 +159,10 @@ static void enqueue_search(const Engine &e, hipStream_t s, Plan *P, const GraphS
     if (D == 1 || v == SL_BYTE) {
         kd_perft1<<<gWork, 256, 0, s>>>(P, last < 0 ? 0 : last);
     } else {
+        // the leaf over the boards with moves (table hits and collapsed
+        // duplicates have none)
+        kd_plan_live<<<1, 64, 0, s>>>(P, last);
+        kd_live<<<gWork, LIVE_WG, 0, s>>>(P, last);
         PG_VARIANT_CALL(v, cap_leaf, s, P, last, gLeaf, e.dt);
     }
     for (int l = lastExpand; l >= 0; l--) {
