@@ -48,7 +48,7 @@ HBM_PEAK_GBS = 8000.0
 # (profiles/k_leaf_fused_kogge_start_d7_d8_c2_d6.md)
 PROFILE_VALU_INSTS_PER_BOARD = 629
 PROFILE_SOURCE = "profiles/k_leaf_fused_kogge_start_d7_d8_c2_d6.md"
-LEAF_KERNEL = "k_leaf_fused<0, false>"
+LEAF_KERNEL = "k_leaf_moves<0>"
 COPY_BYTES = 1 << 30
 COPY_ITERS = 3
 
@@ -82,6 +82,8 @@ def run_step(dev, jobs, variant, leaf_acc=None):
             leaf_acc["ms"] += st["leaf_kernel_ms"]
             leaf_acc["items"] += st["leaf_items"]
             leaf_acc["parents"] += st["leaf_parents"]
+            # the leaf makes its parents from the boards one ply up (the root at depth 3)
+            leaf_acc["grandparents"] += st["nodes"][depth - 3] if depth > 3 else 1
     return leaves
 
 
@@ -279,7 +281,8 @@ def leg_counters(pg, dev=None):
            "lane_utilization": L.get("SQ_THREAD_CYCLES_VALU", 0) / (64.0 * max(1.0, L.get("SQ_ACTIVE_INST_VALU", 0))),
            "lds_insts": L.get("SQ_INSTS_LDS"), "lds_bank_conflict_cycles": L.get("SQ_LDS_BANK_CONFLICT"),
            "salu_insts": L.get("SQ_INSTS_SALU"), "waves": L.get("SQ_WAVES"),
-           "hbm_bytes_per_launch": traffic, "algorithmic_bytes_per_launch": 52.0 * tot["parents"] / tot["launches"],
+           "hbm_bytes_per_launch": traffic,
+           "algorithmic_bytes_per_launch": (6.0 * tot["parents"] + 48.0 * tot["grandparents"]) / tot["launches"],
            "copy_calibration": {"bytes_per_dispatch": COPY_BYTES, "dispatches": nc,
                                 "fetch_size_kib_per_dispatch": C["FETCH_SIZE"] / nc,
                                 "write_size_kib_per_dispatch": C["WRITE_SIZE"] / nc,
@@ -294,9 +297,11 @@ def leg_counters(pg, dev=None):
                              "dual_issue_fraction": sums[k].get("SQ_ACTIVE_INST_VALU2", 0) / sums[k]["SQ_INSTS_VALU"]}
     res["issue_probes"] = probes
     log("counters: %.0f VALU instructions per board, %.3f per CU per cycle (dual issue %.1f%%, lanes active %.1f%%), "
-        "%.1f B HBM per parent" % (res["valu_insts_per_board"], res["valu_wave_insts_per_cu_per_cycle"],
-                                   100 * res["dual_issue_fraction"], 100 * res["lane_utilization"],
-                                   traffic * tot["launches"] / max(1, tot["parents"])))
+        "%.1f B HBM per parent (algorithmic %.1f)" % (res["valu_insts_per_board"], res["valu_wave_insts_per_cu_per_cycle"],
+                                                     100 * res["dual_issue_fraction"], 100 * res["lane_utilization"],
+                                                     traffic * tot["launches"] / max(1, tot["parents"]),
+                                                     res["algorithmic_bytes_per_launch"] * tot["launches"]
+                                                     / max(1, tot["parents"])))
     return res
 
 
@@ -460,7 +465,7 @@ def main():
     # roofline pass: the same step once more on a context that brackets every
     # leaf-kernel launch with HIP events on its launch stream (the host-driven
     # level loop, which launches the same fused leaf as a kernel of its own)
-    acc = {"launches": 0, "ms": 0.0, "items": 0, "parents": 0}
+    acc = {"launches": 0, "ms": 0.0, "items": 0, "parents": 0, "grandparents": 0}
     if args.steps > 0:
         with pg.Device(0, arena_bytes=48 << 30, timing=True) as tdev:
             run_step(tdev, jobs, best)
@@ -470,9 +475,11 @@ def main():
     avg_ms = acc["ms"] / launches
     fused = best != pg.SLIDER_MAGIC_BYTE   # the byte-lookup variant keeps the materialized leaf level
     if fused:
-        # items live in LDS: a 48-byte board and a 4-byte move count per parent
-        bytes_per_launch = 52 * acc["parents"] / launches
-        kname, bnote = "k_leaf_fused", "52 B per parent board (board + move count); the (parent, move) items stay in LDS"
+        # the leaf's parents are made in LDS from a move and a parent index
+        # (6 B each) and the boards one ply up (48 B each); their moves stay in LDS
+        bytes_per_launch = (6 * acc["parents"] + 48 * acc["grandparents"]) / launches
+        kname, bnote = "k_leaf_moves", ("6 B per parent (move + index of the board it is made from) + 48 B per board "
+                                        "one ply up; the parents and their (parent, move) items stay in LDS")
     else:
         bytes_per_launch = (6 * acc["items"] + 48 * acc["parents"]) / launches
         kname, bnote = "k_leaf", "6 B per (parent, move) item + 48 B per parent board"
@@ -513,7 +520,7 @@ def main():
                    "slider": pg.SLIDER_NAMES[best], "exact": True},
         "variants": variants, "roofline": roofline,
         "leaf_totals": {"launches": acc["launches"], "items": acc["items"], "parents": acc["parents"],
-                        "ms": acc["ms"]},
+                        "grandparents": acc["grandparents"], "ms": acc["ms"]},
     }
     if counters is not None:
         line["counters"] = counters

@@ -186,7 +186,8 @@ static __global__ __launch_bounds__(SCAN_WG) void kd_scan_down(Plan *P, int l) {
 }
 
 // allocate level l+1 (children of l) and the transient arrays; a total-only
-// search keeps just boards and counts per level
+// search keeps just boards and counts per level, and for its last expansion
+// (total == 2) only the moves and parents: the leaf makes those children
 __device__ void plan_children(Plan *P, int l, int wantKeys, int wantSet, int total) {
     DLevel &L = P->lv[l];
     DLevel &C = P->lv[l + 1];
@@ -194,6 +195,13 @@ __device__ void plan_children(Plan *P, int l, int wantKeys, int wantSet, int tot
     const u64 m = L.n ? L.off[L.n] : 0;
     if (m == 0 || m > 0xFFFFFFFFULL) {
         if (m) P->overflow = 1;
+        return;
+    }
+    if (total == 2) {
+        L.release = P->top;
+        L.mv = (u16 *)plan_alloc(P, m * 2);
+        L.par = (u32 *)plan_alloc(P, m * 4);
+        if (!P->overflow) C.n = m;
         return;
     }
     C.b = (Pos *)plan_alloc(P, m * sizeof(Pos));
@@ -378,6 +386,17 @@ __global__ __launch_bounds__(LEAF_WG) __attribute__((amdgpu_waves_per_eu(V == SL
     const bool listed = !TOTAL && L.live;
     leaf_tiles<V, !TOTAL>(L.b, L.cnt, listed ? P->liveN : L.n, &P->acc, &P->leafItems, L.perft, L.state,
                           l == 0 ? P->bias : 0, &P->err, &P->ovf, t, listed ? L.live : nullptr);
+}
+
+// perft(3) of the boards of level l summed into the plan's total: the
+// children of level l (its move list) are made, counted and expanded inside
+// the fused leaf (leaf_tiles, MOVES)
+template <int V>
+__global__ __launch_bounds__(LEAF_WG) __attribute__((amdgpu_waves_per_eu(V == SL_KOGGE ? 4 : 5, 8))) void kd_leaf_moves(
+    Plan *P, int l, Tables t) {
+    const DLevel L = P->lv[l];
+    leaf_tiles<V, false, true>(L.b, nullptr, P->lv[l + 1].n, &P->acc, &P->leafItems, nullptr, nullptr, 0, &P->err,
+                               &P->ovf, t, nullptr, L.par, L.mv);
 }
 
 // the plan's total += the move counts of level l (perft(1) of a total-only search)

@@ -65,6 +65,7 @@ struct Engine {
     int scan(const u32 *cnt, u64 n, u64 *off, u64 *total);
     int leaf(int v, const Pos *b, u64 n, const u32 *par, const u16 *mv, u64 m);
     int leaf_fused(int v, const Pos *b, const u32 *cnt, u64 n);
+    int leaf_moves(int v, const Pos *gp, const u32 *par, const u16 *mv, u64 n);
     int bfs(int v, const Pos *boards, u64 n, const u32 *cnt, int r, int ply);
     int split(int v, const Pos *boards, u64 n, const u32 *cnt, int r, int ply);
     int perft(const Pos &root, u32 depth, int v, u64 *out);

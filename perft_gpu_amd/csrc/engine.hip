@@ -130,6 +130,34 @@ template <int V> static void launch_leaf_fused(Engine &c, const Pos *b, const u3
     }
 }
 
+template <int V>
+static void launch_leaf_moves(Engine &c, const Pos *gp, const u32 *par, const u16 *mv, u64 n) {
+    if constexpr (V == SL_BYTE) {
+        (void)c; (void)gp; (void)par; (void)mv; (void)n;
+    } else {
+        k_leaf_moves<V><<<c.leaf_grid(n), LEAF_WG, 0, c.stream>>>(gp, par, mv, n, c.dAcc, c.dItems, c.dErr, c.dOvf,
+                                                                  c.dt);
+    }
+}
+
+// the last three plies in one kernel: the n children of gp (par/mv) are made
+// and counted in the leaf's work-groups
+int Engine::leaf_moves(int v, const Pos *gp, const u32 *par, const u16 *mv, u64 n) {
+    if (timing) HIP_TRY(hipEventRecord(ev0, stream));
+    PG_VARIANT_CALL(v, launch_leaf_moves, *this, gp, par, mv, n);
+    HIP_TRY(hipGetLastError());
+    if (timing) {
+        HIP_TRY(hipEventRecord(ev1, stream));
+        HIP_TRY(hipEventSynchronize(ev1));
+        float ms = 0;
+        HIP_TRY(hipEventElapsedTime(&ms, ev0, ev1));
+        stats.leaf_kernel_ms += ms;
+    }
+    stats.leaf_launches++;
+    stats.leaf_parents += n;
+    return PG_OK;
+}
+
 int Engine::leaf_fused(int v, const Pos *b, const u32 *cnt, u64 n) {
     if (timing) HIP_TRY(hipEventRecord(ev0, stream));
     PG_VARIANT_CALL(v, launch_leaf_fused, *this, b, cnt, n);
@@ -177,13 +205,16 @@ int Engine::bfs(int v, const Pos *boards, u64 n, const u32 *cnt, int r, int ply)
     if (rc) return rc;
     if (total == 0) { top = mark; return PG_OK; }
 
+    // three plies left with the fused leaf: the children are made inside the
+    // leaf from the move list, so only moves and parents are stored
+    const bool leafMoves = r == 3 && leafMode == 0 && v != SL_BYTE;
     Pos *children = nullptr;
     u32 *ccnt = nullptr;
-    if (r > 2) {
+    if (r > 2 && !leafMoves) {
         children = (Pos *)alloc(total * sizeof(Pos));
         ccnt = children ? (u32 *)alloc(total * sizeof(u32)) : nullptr;
     }
-    u16 *mv = (r == 2 || ccnt) ? (u16 *)alloc(total * sizeof(u16)) : nullptr;
+    u16 *mv = (r == 2 || ccnt || leafMoves) ? (u16 *)alloc(total * sizeof(u16)) : nullptr;
     u32 *par = mv ? (u32 *)alloc(total * sizeof(u32)) : nullptr;
     if (!par) {
         top = mark;
@@ -195,6 +226,11 @@ int Engine::bfs(int v, const Pos *boards, u64 n, const u32 *cnt, int r, int ply)
     HIP_TRY(hipGetLastError());
     if (r == 2) {
         rc = leaf(v, boards, n, par, mv, total);
+        top = mark;
+        return rc;
+    }
+    if (leafMoves) {
+        rc = leaf_moves(v, boards, par, mv, total);
         top = mark;
         return rc;
     }

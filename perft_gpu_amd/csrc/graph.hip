@@ -102,6 +102,9 @@ template <int V> static void cap_leaf(hipStream_t s, Plan *P, int l, int g, cons
 template <int V> static void cap_leaf_total(hipStream_t s, Plan *P, int l, int g, const Tables &t) {
     if constexpr (V != SL_BYTE) kd_leaf<V, true><<<g, LEAF_WG, 0, s>>>(P, l, t);
 }
+template <int V> static void cap_leaf_moves(hipStream_t s, Plan *P, int l, int g, const Tables &t) {
+    if constexpr (V != SL_BYTE) kd_leaf_moves<V><<<g, LEAF_WG, 0, s>>>(P, l, t);
+}
 
 // The search of one root when only its perft is wanted: per level just
 // boards and counts (no keys, table or per-board sums), the leaf adds into the
@@ -113,11 +116,18 @@ static void enqueue_total(const Engine &e, hipStream_t s, Plan *P, const GraphSh
     PG_VARIANT_CALL(v, cap_count, s, P, gv, e.dt);
     const int lastExpand = v == SL_BYTE ? D - 2 : D - 3;
     for (int l = 0; l <= lastExpand; l++) {
+        // with the fused leaf the last expansion lists moves only: the leaf
+        // makes and counts those children itself (kd_leaf_moves)
+        const bool movesOnly = v != SL_BYTE && l == lastExpand;
         kd_plan_scan<<<1, 64, 0, s>>>(P, l);
         kd_scan_reduce<<<gScan, SCAN_WG, 0, s>>>(P, l);
-        kd_scan_spine<<<1, 1024, 0, s>>>(P, l, 0, 0, 1);
+        kd_scan_spine<<<1, 1024, 0, s>>>(P, l, 0, 0, movesOnly ? 2 : 1);
         kd_scan_down<<<gScan, SCAN_WG, 0, s>>>(P, l);
         PG_VARIANT_CALL(v, cap_generate, s, P, l, gv, e.dt);
+        if (movesOnly) {
+            PG_VARIANT_CALL(v, cap_leaf_moves, s, P, l, G.leaf, e.dt);
+            return;
+        }
         PG_VARIANT_CALL(v, cap_expand, s, P, l, TTView{}, D - l - 1, (TTCounters *)nullptr, gv, e.dt);
     }
     const int last = lastExpand + 1 < 0 ? 0 : lastExpand + 1;  // deepest materialized level

@@ -300,10 +300,16 @@ enum : u8 { ST_LIVE = 0, ST_HIT = 1, ST_DUP = 2 };
 // boards (state[i] == ST_LIVE when state is given, else a valid board).
 // With idx, the boards are idx[0..n) (live boards with moves, listed by
 // k_live_write) and per[] is written at those indices.
-template <int V, bool PER>
+// MOVES: the n boards are not stored: board j is boards[par[j]] after move
+// mv[j] (the children of the level before, as the generate kernel lists
+// them); the work-group makes each one and counts its moves itself, so the
+// level one above the leaf never goes to HBM.  Sums only (PER = false).
+template <int V, bool PER, bool MOVES = false>
 __device__ __forceinline__ void leaf_tiles(const Pos *__restrict__ boards, const u32 *__restrict__ cnt, u64 n, u64 *acc,
                                            u64 *items, u64 *__restrict__ per, const u8 *__restrict__ state, u64 base,
-                                           u32 *err, u32 *ovf, const Tables &t, const u32 *__restrict__ idx = nullptr) {
+                                           u32 *err, u32 *ovf, const Tables &t, const u32 *__restrict__ idx = nullptr,
+                                           const u32 *__restrict__ par = nullptr, const u16 *__restrict__ mv = nullptr) {
+    static_assert(!(MOVES && PER), "a leaf over moves only sums");
     static_assert(V != SL_BYTE, "the byte-lookup tables leave no LDS for the move window");
     __shared__ Pos sb[LEAF_WG];
     constexpr int LEAF_CAP = leaf_cap<PER>();
@@ -318,8 +324,18 @@ __device__ __forceinline__ void leaf_tiles(const Pos *__restrict__ boards, const
         const u64 t0 = tile * LEAF_WG + tid;
         const bool live = t0 < n;
         const u64 i = idx && live ? (u64)idx[t0] : t0;
-        const u32 c = live ? cnt[i] : 0;
-        if (live) sb[tid] = boards[i];
+        u32 c = 0;
+        if constexpr (MOVES) {
+            if (live) {
+                Pos b = boards[par[t0]];
+                make_any(b, mv[t0], tl);
+                c = count_any<V>(b, tl);
+                sb[tid] = b;
+            }
+        } else {
+            c = live ? cnt[i] : 0;
+            if (live) sb[tid] = boards[i];
+        }
         if (PER) sper[tid] = 0;
         u64 total64;
         const u32 pos = (u32)block_excl_scan<LEAF_WG>(c, &total64);
@@ -369,6 +385,14 @@ __global__ __launch_bounds__(LEAF_WG) __attribute__((amdgpu_waves_per_eu(V == SL
     const Pos *__restrict__ boards, const u32 *__restrict__ cnt, u64 n, u64 *acc, u64 *items, u64 *__restrict__ per,
     u32 *err, u32 *ovf, Tables t, const u32 *__restrict__ idx) {
     leaf_tiles<V, PER>(boards, cnt, n, acc, items, per, nullptr, 0, err, ovf, t, idx);
+}
+
+// the leaf over the children of `gp` listed by par/mv (see leaf_tiles, MOVES)
+template <int V>
+__global__ __launch_bounds__(LEAF_WG) __attribute__((amdgpu_waves_per_eu(V == SL_KOGGE ? 4 : 5, 8))) void k_leaf_moves(
+    const Pos *__restrict__ gp, const u32 *__restrict__ par, const u16 *__restrict__ mv, u64 n, u64 *acc, u64 *items,
+    u32 *err, u32 *ovf, Tables t) {
+    leaf_tiles<V, false, true>(gp, nullptr, n, acc, items, nullptr, nullptr, 0, err, ovf, t, nullptr, par, mv);
 }
 
 // ---------------------------------------------------------------- live list

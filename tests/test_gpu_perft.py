@@ -52,15 +52,17 @@ def test_start_depth8(dev, v):
 
 
 def test_small_arena_forces_splits_and_stays_exact(pg):
-    # perft(7) needs ~250 MB for ply 5 and ~700 MB for the ply-6 items; a
-    # 64 MiB arena forces both levels to split (the graph path finds that its
-    # levels do not fit and hands over to the host-driven loop)
-    with pg.Device(0, arena_bytes=64 << 20) as small:
+    # perft(7) stores ~12 MB for ply 4 and 29 MB of moves and parents for the
+    # 4.9e6 ply-5 boards the leaf makes itself (the byte-lookup variant: 250 MB
+    # of ply-5 boards and 700 MB of ply-6 items); a 16 MiB arena forces the
+    # levels to split (the graph path finds that its levels do not fit and
+    # hands over to the host-driven loop)
+    with pg.Device(0, arena_bytes=16 << 20) as small:
         for v in (0, 3):
             print("split arena start d7 variant", v, flush=True)
             assert small.perft(W.CPW["start"], 7, v) == W.CPW_COUNTS["start"][7]
             st = small.last_stats()
-            assert st["splits"] > 0 and st["peak_arena"] <= 64 << 20
+            assert st["splits"] > 0 and st["peak_arena"] <= 16 << 20
             assert st["nodes"][1:7] == W.CPW_COUNTS["start"][1:7]
 
 
