@@ -44,10 +44,10 @@ METRIC = "perft leaves/sec on 1 MI355X (exact counts); p50/p99 latency of served
 HBM_PEAK_GBS = 8000.0
 # Used only when the counters leg cannot run (no rocprofv3): VALU
 # instructions per counted board (SQ_INSTS_VALU x 64 / boards, one lane per
-# board) of k_leaf_fused<kogge_stone> measured by rocprofv3 on this workload
-# (profiles/k_leaf_fused_kogge_start_d7_d8_c2_d6.md)
-PROFILE_VALU_INSTS_PER_BOARD = 629
-PROFILE_SOURCE = "profiles/k_leaf_fused_kogge_start_d7_d8_c2_d6.md"
+# board) of k_leaf_moves<kogge_stone> measured by rocprofv3 on this workload
+# (profiles/k_leaf_moves_kogge_start_d7_d8_c2_d6.md)
+PROFILE_VALU_INSTS_PER_BOARD = 652
+PROFILE_SOURCE = "profiles/k_leaf_moves_kogge_start_d7_d8_c2_d6.md"
 LEAF_KERNEL = "k_leaf_moves<0>"
 COPY_BYTES = 1 << 30
 COPY_ITERS = 3
