@@ -309,7 +309,8 @@ int pg_device_memory(pg_ctx *ctx, uint64_t *free_bytes, uint64_t *total_bytes);
  * slot's HBM region allow (region bytes per query of each depth are learned
  * from the peak use of that depth's finished batches); depths >= long_depth
  * hold at most long_slots slots, and their kernels at most long_wg_per_cu
- * work-groups per CU, so short queries keep finding free slots and CUs.
+ * work-groups per CU (light_wg_per_cu when few queries wait), so short
+ * queries keep finding free slots and CUs.
  * Zero fields take the defaults in brackets. */
 typedef struct pg_serve_config {
     uint32_t slots;           /* concurrent batches, one stream each [GPU_MAX_HW_QUEUES - 2, 1..4]; each
@@ -322,6 +323,9 @@ typedef struct pg_serve_config {
     uint32_t long_wg_per_cu;  /* work-groups per CU a long batch's kernels keep resident [2] */
     int slider;               /* slider variant [0 = Kogge-Stone; PG_SLIDER_AUTO = the context default] */
     int dedup;                /* collapse identical positions within a level */
+    uint32_t light_wg_per_cu; /* ... while at most light_queue queries wait: a light load, where short
+                                 queries' latency matters more than long ones' throughput [1] */
+    uint32_t light_queue;     /* [16] */
 } pg_serve_config;
 typedef struct pg_serve_result {
     uint64_t id;        /* from pg_serve_submit */

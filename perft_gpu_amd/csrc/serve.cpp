@@ -17,7 +17,10 @@
 //    from the peak bytes of the depth's completed batches (a batch that
 //    still overflows its region is finished in halves by the slot);
 //  * the kernels of a long batch keep at most long_wg_per_cu work-groups per
-//    CU resident, so that short batches on other slots find free CUs.
+//    CU resident, so that short batches on other slots find free CUs; while
+//    at most light_queue queries wait (a light load) the limit is
+//    light_wg_per_cu, lower, since short queries' latency then matters more
+//    than long ones' throughput.
 #include <algorithm>
 #include <chrono>
 #include <cmath>
@@ -128,7 +131,9 @@ int pg_serve_create(pg_ctx *ctx, pg_tt *tt, const pg_serve_config *config, pg_se
     if (!c.long_depth) c.long_depth = 6;
     if (!c.long_slots) c.long_slots = c.slots > 1 ? c.slots / 2 : 1;
     if (!c.long_wg_per_cu) c.long_wg_per_cu = 2;
-    if (c.slots > 16 || c.long_slots > c.slots || c.long_wg_per_cu > 64) return PG_EINVAL;
+    if (!c.light_wg_per_cu) c.light_wg_per_cu = std::min(1u, c.long_wg_per_cu);
+    if (!c.light_queue) c.light_queue = 16;
+    if (c.slots > 16 || c.long_slots > c.slots || c.long_wg_per_cu > 64 || c.light_wg_per_cu > 64) return PG_EINVAL;
     if (!c.bytes_per_slot) {
         // admission by free HBM: 60% of what the device has free now, over the slots
         uint64_t freeB = 0, totalB = 0;
@@ -254,7 +259,10 @@ int pg_serve_poll(pg_serve *s, pg_serve_result *out, uint32_t max, uint32_t *n) 
             q.admitted = admitted;
             s->stage[j] = q.pos;
         }
-        const uint32_t cap = depth >= s->cfg.long_depth ? s->cfg.long_wg_per_cu : 0;
+        uint32_t queued = 0;
+        for (int d = 0; d < kMaxDepth; d++) queued += (uint32_t)s->waiting[d].size();
+        const uint32_t cap = depth < s->cfg.long_depth ? 0
+                             : queued <= s->cfg.light_queue ? s->cfg.light_wg_per_cu : s->cfg.long_wg_per_cu;
         const int rc = pg_server_submit_capped(s->srv, slot, s->stage.data(), count, depth, s->cfg.slider, s->cfg.dedup,
                                                cap);
         const double took = now_ms() - admitted;

@@ -22,7 +22,8 @@ P = ctypes.POINTER
 class ServeConfig(ctypes.Structure):
     _fields_ = [("slots", ctypes.c_uint32), ("bytes_per_slot", u64), ("max_batch", ctypes.c_uint32),
                 ("long_depth", ctypes.c_uint32), ("long_slots", ctypes.c_uint32), ("long_wg_per_cu", ctypes.c_uint32),
-                ("slider", ctypes.c_int), ("dedup", ctypes.c_int)]
+                ("slider", ctypes.c_int), ("dedup", ctypes.c_int), ("light_wg_per_cu", ctypes.c_uint32),
+                ("light_queue", ctypes.c_uint32)]
 
 
 class ServeResult(ctypes.Structure):
@@ -94,10 +95,10 @@ class ServeLoop:
     _close_rank = 0
 
     def __init__(self, dev, tt=None, slots=0, slot_bytes=0, max_batch=1024, long_depth=6, long_slots=0,
-                 long_wg_per_cu=2, slider=SLIDER_KOGGE_STONE, dedup=True):
+                 long_wg_per_cu=2, slider=SLIDER_KOGGE_STONE, dedup=True, light_wg_per_cu=0, light_queue=0):
         L = _sigs()
         self.cfg = ServeConfig(slots, slot_bytes, max_batch, long_depth, long_slots, long_wg_per_cu, slider,
-                               1 if dedup else 0)
+                               1 if dedup else 0, light_wg_per_cu, light_queue)
         h = ctypes.c_void_p()
         _check(L.pg_serve_create(dev._h, tt._h if tt is not None else None, ctypes.byref(self.cfg), ctypes.byref(h)),
                "pg_serve_create")
