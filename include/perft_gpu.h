@@ -323,8 +323,9 @@ typedef struct pg_serve_config {
     uint32_t long_wg_per_cu;  /* work-groups per CU a long batch's kernels keep resident [2] */
     int slider;               /* slider variant [0 = Kogge-Stone; PG_SLIDER_AUTO = the context default] */
     int dedup;                /* collapse identical positions within a level */
-    uint32_t light_wg_per_cu; /* ... while at most light_queue queries wait: a light load, where short
-                                 queries' latency matters more than long ones' throughput [1] */
+    uint32_t light_wg_per_cu; /* ... while at most light_queue queries wait and short queries are about
+                                 (waiting, running or submitted in the last 10 ms): a light load, where
+                                 their latency matters more than long ones' throughput [1] */
     uint32_t light_queue;     /* [16] */
 } pg_serve_config;
 typedef struct pg_serve_result {

@@ -18,9 +18,10 @@
 //    still overflows its region is finished in halves by the slot);
 //  * the kernels of a long batch keep at most long_wg_per_cu work-groups per
 //    CU resident, so that short batches on other slots find free CUs; while
-//    at most light_queue queries wait (a light load) the limit is
-//    light_wg_per_cu, lower, since short queries' latency then matters more
-//    than long ones' throughput.
+//    at most light_queue queries wait (a light load) and short queries are
+//    about (one waiting or running, or one submitted in the last
+//    kShortRecentMs) the limit is light_wg_per_cu, lower, since short
+//    queries' latency then matters more than long ones' throughput.
 #include <algorithm>
 #include <chrono>
 #include <cmath>
@@ -35,6 +36,7 @@
 namespace {
 
 constexpr int kMaxDepth = 16;
+constexpr double kShortRecentMs = 10.0;  // a short query this recent may have company on its way
 
 double now_ms() {
     return std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now().time_since_epoch()).count();
@@ -82,6 +84,7 @@ struct pg_serve {
     std::vector<pg_pos48> stage;
     pg_serve_stats st{};
     double firstSubmit = -1, lastDone = 0;
+    double lastShort = -1e300;  // submit time of the latest short query
 };
 
 extern "C" {
@@ -179,6 +182,7 @@ int pg_serve_submit(pg_serve *s, const pg_pos48 *pos, uint32_t depth, uint64_t t
     if (s->firstSubmit < 0) s->firstSubmit = q.arrived;
     s->queries.emplace(q.id, q);
     s->waiting[depth].push_back(q.id);
+    if (depth < s->cfg.long_depth) s->lastShort = q.arrived;
     s->st.submitted++;
     if (id) *id = q.id;
     return PG_OK;
@@ -260,9 +264,16 @@ int pg_serve_poll(pg_serve *s, pg_serve_result *out, uint32_t max, uint32_t *n) 
             s->stage[j] = q.pos;
         }
         uint32_t queued = 0;
-        for (int d = 0; d < kMaxDepth; d++) queued += (uint32_t)s->waiting[d].size();
-        const uint32_t cap = depth < s->cfg.long_depth ? 0
-                             : queued <= s->cfg.light_queue ? s->cfg.light_wg_per_cu : s->cfg.long_wg_per_cu;
+        bool shortsAbout = admitted - s->lastShort < kShortRecentMs;
+        for (int d = 0; d < kMaxDepth; d++) {
+            queued += (uint32_t)s->waiting[d].size();
+            if (d < (int)s->cfg.long_depth && !s->waiting[d].empty()) shortsAbout = true;
+        }
+        for (auto &x : s->slots)
+            if (x.busy && x.depth < s->cfg.long_depth) shortsAbout = true;
+        const uint32_t cap = depth < s->cfg.long_depth                       ? 0
+                             : shortsAbout && queued <= s->cfg.light_queue ? s->cfg.light_wg_per_cu
+                                                                          : s->cfg.long_wg_per_cu;
         const int rc = pg_server_submit_capped(s->srv, slot, s->stage.data(), count, depth, s->cfg.slider, s->cfg.dedup,
                                                cap);
         const double took = now_ms() - admitted;
