@@ -134,11 +134,15 @@ PG_HD bool same_board(const Pos &a, const Pos &b) {
            a.bishopQueens == b.bishopQueens && a.rookQueens == b.rookQueens && a.kings == b.kings;
 }
 
-// In-level duplicate removal: each live child inserts its index into an
-// open-addressing set keyed by its Zobrist key; an equal board already in
-// the set becomes its owner and the child is resolved from it later.  The
-// set has at least twice as many slots as children, so a probe sequence
-// always meets an empty slot; the loop is also capped at the set size.
+// In-level duplicate removal of the table search: each live child inserts
+// its index into an open-addressing set keyed by its Zobrist key; a child
+// already in the set with the same 128-bit key becomes its owner and the
+// child is resolved from it later.  Equal keys stand for equal positions
+// here exactly as they do for a table hit, which is accepted on the same
+// 128-bit key (the searches without a table compare boards too: kd_dedup).
+// The set has at least twice as many slots as children, so a probe
+// sequence always meets an empty slot; the loop is also capped at the set
+// size.
 // one child after its first probe: follow the probe sequence until the child
 // owns a slot or finds its equal
 __device__ __forceinline__ void dedup_finish(u64 i, bool live, u64 s, u32 cur, Pos *__restrict__ children,
@@ -150,7 +154,7 @@ __device__ __forceinline__ void dedup_finish(u64 i, bool live, u64 s, u32 cur, P
         for (u64 step = 0;; step++) {
             if (cur == 0) break;  // claimed: this child owns the position
             const u32 o = cur - 1;
-            if (ckeys[o].lo == ckeys[i].lo && ckeys[o].hi == ckeys[i].hi && same_board(children[o], children[i])) {
+            if (ckeys[o].lo == ckeys[i].lo && ckeys[o].hi == ckeys[i].hi) {
                 dup[i] = o;
                 isDup = true;
                 break;
