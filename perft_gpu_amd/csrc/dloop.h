@@ -291,13 +291,13 @@ static __global__ __launch_bounds__(256) void kd_clear_set(Plan *P, int l) {
 // one child after its first probe (kd_dedup): follow the probe sequence
 // until the child owns a slot or finds its equal
 __device__ __forceinline__ void kd_dedup_finish(Plan *P, const DLevel &L, const DLevel &C, u64 i, bool live, u64 s,
-                                                u32 cur, TTCounters *ctr, int rc) {
+                                                u32 cur, TTCounters *ctr, int rc, int keyOnly) {
     bool isDup = false;
     if (live) {
         for (u64 step = 0;; step++) {
             if (cur == 0) break;
             const u32 o = cur - 1;
-            if (C.k[o].lo == C.k[i].lo && C.k[o].hi == C.k[i].hi && same_board(C.b[o], C.b[i])) {
+            if (C.k[o].lo == C.k[i].lo && C.k[o].hi == C.k[i].hi && (keyOnly || same_board(C.b[o], C.b[i]))) {
                 C.dup[i] = o;
                 isDup = true;
                 break;
@@ -315,8 +315,10 @@ __device__ __forceinline__ void kd_dedup_finish(Plan *P, const DLevel &L, const 
 }
 
 // in-level duplicates of level l+1 (as k_dedup: two children per thread,
-// both first probes in flight)
-static __global__ __launch_bounds__(256) void kd_dedup(Plan *P, int l, TTCounters *ctr, int rc) {
+// both first probes in flight).  keyOnly (a search with a table, which
+// accepts hits on the 128-bit key): equal keys are equal positions; without
+// a table the boards are compared as well.
+static __global__ __launch_bounds__(256) void kd_dedup(Plan *P, int l, TTCounters *ctr, int rc, int keyOnly) {
     const DLevel L = P->lv[l];
     const DLevel C = P->lv[l + 1];
     if (!L.set) return;
@@ -328,8 +330,8 @@ static __global__ __launch_bounds__(256) void kd_dedup(Plan *P, int l, TTCounter
         const u64 s0 = (C.k[i0].lo >> 7) & L.setMask, s1 = l1 ? (C.k[i1].lo >> 7) & L.setMask : 0;
         const u32 c0 = l0 ? atomicCAS(&L.set[s0], 0u, (u32)i0 + 1) : 0;
         const u32 c1 = l1 ? atomicCAS(&L.set[s1], 0u, (u32)i1 + 1) : 0;
-        kd_dedup_finish(P, L, C, i0, l0, s0, c0, ctr, rc);
-        if (has1) kd_dedup_finish(P, L, C, i1, l1, s1, c1, ctr, rc);
+        kd_dedup_finish(P, L, C, i0, l0, s0, c0, ctr, rc, keyOnly);
+        if (has1) kd_dedup_finish(P, L, C, i1, l1, s1, c1, ctr, rc, keyOnly);
     }
 }
 

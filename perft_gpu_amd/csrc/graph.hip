@@ -162,7 +162,7 @@ static void enqueue_search(const Engine &e, hipStream_t s, Plan *P, const GraphS
         PG_VARIANT_CALL(v, cap_expand, s, P, l, tt, rc, ctr, gv, e.dt);
         if (wantSet) {
             kd_clear_set<<<gWork, 256, 0, s>>>(P, l);
-            kd_dedup<<<gWork, 256, 0, s>>>(P, l, ctr, rc);
+            kd_dedup<<<gWork, 256, 0, s>>>(P, l, ctr, rc, sh.tt != nullptr);
         }
     }
     const int last = lastExpand + 1;  // deepest materialized level
