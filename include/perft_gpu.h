@@ -296,6 +296,13 @@ int pg_server_poll(pg_server *srv, uint32_t slot, uint64_t *out, pg_batch_stats 
 int pg_server_submit_capped(pg_server *srv, uint32_t slot, const pg_pos48 *pos, uint32_t n, uint32_t depth,
                             int slider, int dedup, uint32_t wg_per_cu);
 int pg_server_slots(pg_server *srv, uint32_t *slots, uint64_t *bytes_per_slot);
+/* Capture ahead of time, on every idle slot, the graphs that batches of
+ * `depth` with 1..max_roots roots and this slider / dedup / wg_per_cu will
+ * replay (a graph is captured on a slot's first batch of each shape, which
+ * holds up the caller for a millisecond or two).  *graphs (may be NULL)
+ * gets the number captured. */
+int pg_server_prepare(pg_server *srv, uint32_t depth, uint32_t max_roots, int slider, int dedup, uint32_t wg_per_cu,
+                      uint32_t *graphs);
 /* hipMemGetInfo of the context's device (admission by free HBM). */
 int pg_device_memory(pg_ctx *ctx, uint64_t *free_bytes, uint64_t *total_bytes);
 

@@ -722,6 +722,36 @@ int pg_server_poll(pg_server *s, uint32_t slot, uint64_t *out, pg_batch_stats *s
     return 1;
 }
 
+int pg_server_prepare(pg_server *s, uint32_t depth, uint32_t max_roots, int slider, int dedup, uint32_t wg_per_cu,
+                      uint32_t *graphs) {
+    if (graphs) *graphs = 0;
+    if (!s || depth < 1 || depth >= TT_MAX_DEPTH || max_roots == 0 || wg_per_cu > 64) return PG_EINVAL;
+    Engine &e = *s->eng;
+    const int v = slider == PG_SLIDER_AUTO ? e.slider : slider;
+    if (v < 0 || v >= SL_COUNT) return PG_EINVAL;
+    (void)hipSetDevice(e.device);
+    // the grids of the batch sizes 1..max_roots (grid_limit takes few values)
+    std::vector<int> limits;
+    for (uint32_t n = 1; n <= max_roots; n++) {
+        const int g = grid_limit(e, n, (int)depth);
+        if (std::find(limits.begin(), limits.end(), g) == limits.end()) limits.push_back(g);
+    }
+    uint32_t made = 0;
+    for (auto &ss : s->slots) {
+        if (ss.busy) continue;
+        for (const int g : limits) {
+            GraphShape sh{(int)depth, v, 1, dedup ? 1 : 0, e.bpc, s->tt, 0, (int)wg_per_cu, g};
+            const size_t before = ss.slot.graphs.size();
+            hipGraphExec_t exec;
+            const int rc = get_graph(e, ss.slot, sh, &exec);
+            if (rc) return rc;
+            made += (uint32_t)(ss.slot.graphs.size() - before);
+        }
+    }
+    if (graphs) *graphs = made;
+    return PG_OK;
+}
+
 int pg_server_slots(pg_server *s, uint32_t *nslots, uint64_t *bytes_per_slot) {
     if (!s) return PG_EINVAL;
     if (nslots) *nslots = (uint32_t)s->slots.size();

@@ -155,6 +155,24 @@ int pg_serve_create(pg_ctx *ctx, pg_tt *tt, const pg_serve_config *config, pg_se
     s->regionBytes = c.bytes_per_slot;
     s->slots.resize(c.slots);
     for (int d = 0; d < kMaxDepth; d++) s->est[d] = initial_estimate((uint32_t)d);
+    // every graph shape of the short depths and the first long ones captured
+    // now: a capture during the run holds up admission for a millisecond or
+    // two, long enough to show in short queries' tails
+    uint64_t prepared = 0;
+    for (uint32_t d = 1; d <= std::min<uint32_t>(c.long_depth + 1, kMaxDepth - 1) && rc == PG_OK; d++) {
+        const uint32_t caps[2] = {d < c.long_depth ? 0u : c.light_wg_per_cu, c.long_wg_per_cu};
+        for (int k = 0; k < (d < c.long_depth ? 1 : 2) && rc == PG_OK; k++) {
+            uint32_t made = 0;
+            rc = pg_server_prepare(s->srv, d, c.max_batch, c.slider, c.dedup, caps[k], &made);
+            prepared += made;
+        }
+    }
+    if (rc) {
+        pg_serve_destroy(s);
+        return rc;
+    }
+    s->st.graphs = prepared;
+    for (auto &sl : s->slots) sl.graphs = prepared / c.slots;
     s->results.resize(c.max_batch);
     s->stage.resize(c.max_batch);
     *out = s;
