@@ -366,6 +366,11 @@ int pg_serve_get_stats(pg_serve *srv, pg_serve_stats *out);
 int pg_serve_policy(const uint32_t waiting[16], const double oldest[16], uint32_t busy_long, uint32_t idle_slots,
                     const pg_serve_config *cfg, uint64_t region_bytes, const double est_bytes[16], uint32_t *depth,
                     uint32_t *count);
+/* The resident work-group cap of the next batch (host only, no device; 0 =
+ * uncapped): 0 below long_depth; light_wg_per_cu while at most light_queue
+ * queries wait and short queries are about (shorts_about != 0); else
+ * long_wg_per_cu.  cfg's fields are used as given (no defaults filled). */
+uint32_t pg_serve_wg_cap(const pg_serve_config *cfg, uint32_t depth, uint32_t queued, int shorts_about);
 
 /* ---------------------------------------------------------- diagnostics */
 /* Measurement reference points (no reference counterpart; the reference's

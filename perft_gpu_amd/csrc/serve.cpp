@@ -114,6 +114,11 @@ int pg_serve_policy(const uint32_t waiting[16], const double oldest[16], uint32_
     return 1;
 }
 
+uint32_t pg_serve_wg_cap(const pg_serve_config *cfg, uint32_t depth, uint32_t queued, int shorts_about) {
+    if (!cfg || depth < cfg->long_depth) return 0;
+    return shorts_about && queued <= cfg->light_queue ? cfg->light_wg_per_cu : cfg->long_wg_per_cu;
+}
+
 void pg_serve_destroy(pg_serve *s);
 
 int pg_serve_create(pg_ctx *ctx, pg_tt *tt, const pg_serve_config *config, pg_serve **out) {
@@ -289,9 +294,7 @@ int pg_serve_poll(pg_serve *s, pg_serve_result *out, uint32_t max, uint32_t *n) 
         }
         for (auto &x : s->slots)
             if (x.busy && x.depth < s->cfg.long_depth) shortsAbout = true;
-        const uint32_t cap = depth < s->cfg.long_depth                       ? 0
-                             : shortsAbout && queued <= s->cfg.light_queue ? s->cfg.light_wg_per_cu
-                                                                          : s->cfg.long_wg_per_cu;
+        const uint32_t cap = pg_serve_wg_cap(&s->cfg, depth, queued, shortsAbout);
         const int rc = pg_server_submit_capped(s->srv, slot, s->stage.data(), count, depth, s->cfg.slider, s->cfg.dedup,
                                                cap);
         const double took = now_ms() - admitted;

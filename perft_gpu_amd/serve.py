@@ -68,6 +68,7 @@ def _sigs():
         ("pg_serve_policy", ctypes.c_int, [P(ctypes.c_uint32), P(ctypes.c_double), ctypes.c_uint32, ctypes.c_uint32,
                                            P(ServeConfig), u64, P(ctypes.c_double), P(ctypes.c_uint32),
                                            P(ctypes.c_uint32)]),
+        ("pg_serve_wg_cap", ctypes.c_uint32, [P(ServeConfig), ctypes.c_uint32, ctypes.c_uint32, ctypes.c_int]),
     ]:
         f = getattr(L, name)
         f.restype = res
@@ -88,6 +89,12 @@ def policy(waiting, oldest, busy_long, idle_slots, region_bytes, est_bytes, **cf
                            ctypes.byref(k))
     _check(rc if rc < 0 else 0, "pg_serve_policy")
     return (int(d.value), int(k.value)) if rc == 1 else None
+
+
+def wg_cap(depth, queued, shorts_about, **cfg):
+    """pg_serve_wg_cap: resident work-groups per CU for the next batch (0 = uncapped)."""
+    c = ServeConfig(**cfg)
+    return int(_sigs().pg_serve_wg_cap(ctypes.byref(c), depth, queued, 1 if shorts_about else 0))
 
 
 class ServeLoop:

@@ -2,7 +2,8 @@
 device): the depth whose oldest query arrived first goes next; long depths
 keep to their share of the slots; a batch takes as many queries as
 max_batch and the slot's region (by the per-depth estimate) allow, and at
-least one."""
+least one; and the work-group cap a long batch runs under
+(pg_serve_wg_cap)."""
 from perft_gpu_amd.serve import policy
 
 CFG = dict(slots=4, max_batch=1024, long_depth=6, long_slots=2, long_wg_per_cu=2)
@@ -45,3 +46,14 @@ def test_batch_size_bounded_by_region_and_max_batch():
 def test_nothing_to_start():
     assert pick({3: 5}, {3: 0.0}, idle=0) is None
     assert pick({}, {}) is None
+
+
+def test_long_batch_work_group_cap():
+    from perft_gpu_amd.serve import wg_cap
+    c = dict(long_depth=6, long_wg_per_cu=2, light_wg_per_cu=1, light_queue=16)
+    assert wg_cap(5, 0, True, **c) == 0                   # short batches run uncapped
+    assert wg_cap(6, 3, True, **c) == 1                   # light load with short queries about
+    assert wg_cap(6, 16, True, **c) == 1
+    assert wg_cap(6, 17, True, **c) == 2                  # a queue past light_queue: throughput
+    assert wg_cap(7, 0, False, **c) == 2                  # no short queries about
+    assert wg_cap(9, 0, True, **dict(c, light_wg_per_cu=3)) == 3
