@@ -131,29 +131,41 @@ def leg_batch(pg, dev):
             "speedup_vs_plain_loop": loop / dt, "exact": True}
 
 
-def leg_tt(pg, dev, sizes=(16, 64), depths=(9, 10)):
+def leg_tt(pg, sizes=(16, 64), depths=(9, 10)):
+    """Table searches on a context of their own, whose arena is what the
+    largest table leaves free (pg.table_search_arena): fewer level splits,
+    so more equal boards are collapsed before the leaf."""
     from perft_gpu_amd import workloads as W
     out = []
-    for gb in sizes:
-        with pg.TranspositionTable(dev, gb << 30) as tt:
-            for d in depths:
-                t = time.perf_counter()
-                n = tt.perft(pg.START_FEN, d, 0)
-                dt = time.perf_counter() - t
-                if n != W.CPW_COUNTS["start"][d]:
-                    raise SystemExit("MISMATCH TT perft(%d)" % d)
-                st = dev.last_stats()
-                out.append({"tt_gb": gb, "depth": d, "perft": n, "s": dt, "effective_leaves_per_s": n / dt,
-                            "peak_arena_gb": st["peak_arena"] / 1e9, "boards_per_ply": st["nodes"][1:d],
-                            "leaf_items": st["leaf_items"], "splits": st["splits"]})
-                log("tt %d GB d%d %.3f s" % (gb, d, dt))
-            c = tt.counters()
-            out[-1]["counters_by_remaining_depth"] = {
-                int(r): {"probes": int(c[r][0]), "hits": int(c[r][1]), "stores": int(c[r][2]),
-                         "in_level_duplicates": int(c[r][3])} for r in range(2, max(depths) + 1) if c[r].any()}
+    arena = pg.table_search_arena(max(sizes) << 30) or pg.DEFAULT_ARENA
+    with pg.Device(0, arena_bytes=arena) as dev:
+        for gb in sizes:
+            out.extend(tt_runs(pg, W, dev, gb, depths))
     return {"workload": "start position, TT perft(%s) with %s GB tables, each table cleared at creation and "
                         "filled by the depths before" % ("/".join(map(str, depths)), "/".join(map(str, sizes))),
-            "runs": out, "exact": True}
+            "arena_gib": arena / (1 << 30), "runs": out, "exact": True}
+
+
+def tt_runs(pg, W, dev, gb, depths):
+    """TT perft of the start position at each depth on one new table of gb GB."""
+    out = []
+    with pg.TranspositionTable(dev, gb << 30) as tt:
+        for d in depths:
+            t = time.perf_counter()
+            n = tt.perft(pg.START_FEN, d, 0)
+            dt = time.perf_counter() - t
+            if n != W.CPW_COUNTS["start"][d]:
+                raise SystemExit("MISMATCH TT perft(%d)" % d)
+            st = dev.last_stats()
+            out.append({"tt_gb": gb, "depth": d, "perft": n, "s": dt, "effective_leaves_per_s": n / dt,
+                        "peak_arena_gb": st["peak_arena"] / 1e9, "boards_per_ply": st["nodes"][1:d],
+                        "leaf_items": st["leaf_items"], "splits": st["splits"]})
+            log("tt %d GB d%d %.3f s" % (gb, d, dt))
+        c = tt.counters()
+        out[-1]["counters_by_remaining_depth"] = {
+            int(r): {"probes": int(c[r][0]), "hits": int(c[r][1]), "stores": int(c[r][2]),
+                     "in_level_duplicates": int(c[r][3])} for r in range(2, max(depths) + 1) if c[r].any()}
+    return out
 
 
 def leg_calib(pg, dev):
@@ -528,7 +540,7 @@ def main():
         if leg == "counters":
             continue
         if leg == "tt":
-            line["tt"] = leg_tt(pg, dev, tuple(int(x) for x in args.tt_gb.split(",")),
+            line["tt"] = leg_tt(pg, tuple(int(x) for x in args.tt_gb.split(",")),
                                 tuple(int(x) for x in args.tt_depths.split(",")))
         else:
             line[leg] = {"batch": leg_batch, "serve": leg_serve, "calib": leg_calib,

@@ -139,6 +139,7 @@ def lib():
         "pg_device_slider_attacks": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.c_void_p,
                                                     ctypes.c_void_p, ctypes.c_uint32, ctypes.c_void_p]),
         "pg_device_copy": (ctypes.c_int, [ctypes.c_void_p, u64, ctypes.c_uint32, P(ctypes.c_double)]),
+        "pg_device_memory": (ctypes.c_int, [ctypes.c_void_p, P(u64), P(u64)]),
         "pg_device_scan": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, u64, ctypes.c_uint32, ctypes.c_void_p]),
         "pg_device_issue_probe": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.c_uint32, P(ctypes.c_double),
                                                  P(u64), P(ctypes.c_uint32), P(ctypes.c_uint32)]),
@@ -414,6 +415,12 @@ class Device:
         _check(lib().pg_device_scan(self._h, _ptr(c), len(c), misalign, _ptr(off)), "pg_device_scan")
         return off
 
+    def memory(self):
+        """(free, total) bytes of the device's HBM (hipMemGetInfo)."""
+        free, total = u64(), u64()
+        _check(lib().pg_device_memory(self._h, ctypes.byref(free), ctypes.byref(total)), "pg_device_memory")
+        return int(free.value), int(total.value)
+
     def copy_bandwidth(self, nbytes, iters=10):
         """Mean time (ms) and GB/s (read + write) of a plain nbytes copy."""
         ms = ctypes.c_double()
@@ -511,6 +518,29 @@ def divide(dev, pos, depth, slider=SLIDER_AUTO, tt=None):
         counts = [(tt.perft(pos_from_words(k), depth - 1, slider) if tt is not None
                    else dev.perft(pos_from_words(k), depth - 1, slider)) for k in kids]
     return {m: int(c) for m, c in zip(moves, counts)}
+
+
+# Tree arena of a context that runs table searches.  A deep table search
+# expands levels of billions of boards (start-position perft(10): 2.6e9 at
+# ply 8, about 85 B each with key, perft, count, state, owner, move and
+# parent); a level that does not fit is split, and equal boards in different
+# parts are not collapsed, so a larger arena means less leaf work: start
+# perft(10) with a 16 GB table takes 1.05 s on 48 GiB (14 splits) and 0.97 s
+# on 144 GiB (2), perft(11) 13.0 s on 48 GiB and 12.1 s on 160 GiB; beyond
+# that the larger collapse set costs what the leaf saves.
+DEFAULT_ARENA = 48 << 30     # the library's default (pg_config.arena_bytes = 0)
+TABLE_ARENA_MAX = 160 << 30
+TABLE_ARENA_RESERVE = 16 << 30
+
+
+def table_search_arena(table_bytes, device=0):
+    """Arena bytes for a context that will hold a table of table_bytes: the
+    HBM free now less the table and TABLE_ARENA_RESERVE, at most
+    TABLE_ARENA_MAX; 0 (the library default) when that is not larger."""
+    with Device(device, arena_bytes=64 << 20) as probe:
+        free, _ = probe.memory()
+    a = min(TABLE_ARENA_MAX, free - int(table_bytes) - TABLE_ARENA_RESERVE)
+    return a if a > DEFAULT_ARENA else 0
 
 
 def perft_deep(dev, pos, depth, launch_depth=10, divided_depth=10, tt=None, slider=SLIDER_AUTO, on_divide=None,

@@ -45,6 +45,9 @@ def main(argv=None):
     ap.add_argument("num_gpus", nargs="?", type=int, default=1)
     ap.add_argument("launch_depth", nargs="?", type=int, default=None)
     ap.add_argument("--tt", type=float, default=0.0, help="transposition table size in GB (0 = no TT)")
+    ap.add_argument("--arena", type=float, default=0.0,
+                    help="tree arena in GiB (0 = the library default; with --tt, as much as the table leaves free, "
+                         "up to 160 GiB)")
     ap.add_argument("--slider", type=int, default=pg.SLIDER_AUTO)
     ap.add_argument("--min-depth", type=int, default=3)
     ap.add_argument("--divide", action="store_true", help="print perft(depth-1) per root move at the last depth")
@@ -70,7 +73,7 @@ def main(argv=None):
     print("No of GPUs detected: 1")
     print(pg.pos_to_fen(pos))
     launch = args.launch_depth if args.launch_depth else DEFAULT_LAUNCH_DEPTH
-    with pg.Device(0) as dev:
+    with pg.Device(0, arena_bytes=arena_bytes(args)) as dev:
         tt = pg.TranspositionTable(dev, int(args.tt * (1 << 30))) if args.tt > 0 else None
         if tt and args.tt_file and os.path.exists(args.tt_file):
             tt.load(args.tt_file)
@@ -100,6 +103,13 @@ def main(argv=None):
     return 0
 
 
+def arena_bytes(args):
+    """--arena, or for a table search the arena the table leaves room for."""
+    if args.arena > 0:
+        return int(args.arena * (1 << 30))
+    return pg.table_search_arena(int(args.tt * (1 << 30))) if args.tt > 0 else 0
+
+
 def divided_perft(dev, tt, pos, depth, launch, slider, disk=None):
     """dividedPerft (launcher.h:1392-1450): the divided lines of every host
     node deeper than DIVIDED_PERFT_DEPTH, then the total with wall and GPU
@@ -121,7 +131,7 @@ def divided_perft(dev, tt, pos, depth, launch, slider, disk=None):
 
 def records(args):
     print("filename of op: %s.op" % args.records)
-    with pg.Device(0) as dev:
+    with pg.Device(0, arena_bytes=int(args.arena * (1 << 30))) as dev:
         tt = pg.TranspositionTable(dev, int(args.tt * (1 << 30))) if args.tt > 0 else None
         try:
             t0 = time.perf_counter()

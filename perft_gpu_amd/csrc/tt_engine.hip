@@ -3,10 +3,11 @@
 // Replaces perft_bb_gpu_simple_hash (perft_bb.h:1242-1589): per level, the
 // children are made with the incremental 128-bit key, the table of their
 // remaining depth is probed (a hit resolves the child), identical children
-// are collapsed to one owner (FIND_DUPLICATES_IN_BFS, perft_bb.h:788-846,
-// here exact: owners are compared board to board), the live children are
-// expanded recursively, and on the way up every computed perft is stored and
-// summed into its parent (calcPerftNFromPerftNminus1, perft_bb.h:849-961).
+// are collapsed to one owner (FIND_DUPLICATES_IN_BFS, perft_bb.h:788-846;
+// here on the full 128-bit key, the same test a table hit passes), the live
+// children are expanded recursively, and on the way up every computed perft
+// is stored and summed into its parent (calcPerftNFromPerftNminus1,
+// perft_bb.h:849-961).
 // A level that does not fit the arena is split in halves; the first half's
 // stores then serve the second half.
 #include <algorithm>

@@ -92,11 +92,20 @@ def test_tt_depth9_equals_no_tt_path(dev, pg):
 @pytest.mark.parametrize("gb", [16, 64])
 def test_tt_depth10_with_16_and_64_gb(pg, gb):
     """TT perft(10) of the start position with 16 GB and 64 GB tables equals
-    the published 69,352,859,712,417 (perft(9) on the same table first)."""
-    with pg.Device(0, arena_bytes=48 << 30) as d, pg.TranspositionTable(d, gb << 30) as big:
+    the published 69,352,859,712,417 (perft(9) on the same table first).  The
+    16 GB table runs on a 48 GiB arena (14 level splits), the 64 GB one on
+    the arena the table leaves free (pg.table_search_arena: one or two)."""
+    arena = 48 << 30 if gb == 16 else (pg.table_search_arena(gb << 30) or pg.DEFAULT_ARENA)
+    with pg.Device(0, arena_bytes=arena) as d, pg.TranspositionTable(d, gb << 30) as big:
         for depth in (9, 10):
-            print("tt %d GB d%d" % (gb, depth), flush=True)
+            print("tt %d GB d%d arena %.0f GiB" % (gb, depth, arena / (1 << 30)), flush=True)
             assert big.perft(W.CPW["start"], depth, 0) == W.CPW_COUNTS["start"][depth]
+
+
+def test_table_search_arena(pg):
+    a = pg.table_search_arena(1 << 30)
+    assert a == 0 or pg.DEFAULT_ARENA < a <= pg.TABLE_ARENA_MAX
+    assert pg.table_search_arena(1 << 50) == 0  # a table larger than the device: the library default
 
 
 def test_tt_checkpoint_resume(dev, pg, tmp_path):

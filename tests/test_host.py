@@ -226,3 +226,14 @@ def test_cpu_perft_entry_point(pg):
             assert pg.cpu_perft(p, d) == want, (item["fen"], d)
     with pytest.raises(pg.PerftError):
         pg.cpu_perft(W.CPW["start"], 16)
+
+
+def test_cli_arena_option():
+    """--arena in GiB wins; without a table the library default (0) stands
+    (a table search sizes its arena on the device, covered on the GPU)."""
+    import argparse
+    from perft_gpu_amd import cli
+    ns = lambda **k: argparse.Namespace(**dict(dict(arena=0.0, tt=0.0), **k))
+    assert cli.arena_bytes(ns()) == 0
+    assert cli.arena_bytes(ns(arena=2.0)) == 2 << 30
+    assert cli.arena_bytes(ns(arena=1.5, tt=16.0)) == 3 << 29
