@@ -268,6 +268,12 @@ typedef struct pg_batch_stats {
  * pos[i]. */
 int pg_perft_batch(pg_ctx *ctx, const pg_pos48 *pos, uint32_t n, uint32_t depth, int slider, int dedup,
                    int blocks_per_cu, pg_tt *tt, uint64_t *out, pg_batch_stats *stats);
+/* Roots with a depth each (the same reference counterpart): one
+ * pg_perft_batch per depth present, shallowest first; out[i] = perft of
+ * pos[i] to depths[i].  stats sums the calls (peak_bytes: their maximum,
+ * total_ms: the whole call). */
+int pg_perft_batch_mixed(pg_ctx *ctx, const pg_pos48 *pos, const uint32_t *depths, uint32_t n, int slider, int dedup,
+                         int blocks_per_cu, pg_tt *tt, uint64_t *out, pg_batch_stats *stats);
 /* Work-groups per kernel of later pg_perft_batch calls (the threads that
  * share a batch's positions): -1 (default) sized to the batch -- its largest
  * level holds at most about n * 48^(depth-2) boards, 256 per work-group,

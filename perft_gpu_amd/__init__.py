@@ -166,6 +166,9 @@ def lib():
         "pg_perft_batch": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_uint32, ctypes.c_uint32,
                                           ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p,
                                           P(BatchStats)]),
+        "pg_perft_batch_mixed": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_uint32,
+                                                ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_void_p,
+                                                ctypes.c_void_p, P(BatchStats)]),
     }
     for name, (res, args) in sig.items():
         f = getattr(L, name)
@@ -372,6 +375,20 @@ class Device:
         st = BatchStats()
         _check(lib().pg_perft_batch(self._h, _ptr(arr), len(arr), depth, slider, 1 if dedup else 0, blocks_per_cu,
                                     tt._h if tt is not None else None, _ptr(out), ctypes.byref(st)), "pg_perft_batch")
+        return out, st.as_dict()
+
+    def perft_batch_mixed(self, positions, depths, slider=SLIDER_AUTO, dedup=True, blocks_per_cu=0, tt=None):
+        """perft(depths[i]) of every position i, one graph launch per depth
+        present (pg_perft_batch_mixed); returns (counts, stats)."""
+        arr = positions_array(positions)
+        d = np.ascontiguousarray(depths, dtype=np.uint32)
+        if len(d) != len(arr):
+            raise ValueError("one depth per position")
+        out = np.zeros(len(arr), dtype=np.uint64)
+        st = BatchStats()
+        _check(lib().pg_perft_batch_mixed(self._h, _ptr(arr), _ptr(d), len(arr), slider, 1 if dedup else 0,
+                                          blocks_per_cu, tt._h if tt is not None else None, _ptr(out),
+                                          ctypes.byref(st)), "pg_perft_batch_mixed")
         return out, st.as_dict()
 
     def count_moves(self, positions, slider=SLIDER_AUTO):

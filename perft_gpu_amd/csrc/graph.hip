@@ -11,6 +11,7 @@
 #include <chrono>
 #include <cstring>
 #include <numeric>
+#include <string>
 #include <vector>
 #include "engine.h"
 #include "dloop.h"
@@ -458,6 +459,52 @@ int pg_perft_batch(pg_ctx *ctx, const pg_pos48 *pos, uint32_t n, uint32_t depth,
     }
     return batch_perft(*e, *e->batchSlot, t, roots.data(), n, depth, v, dedup, blocks_per_cu,
                        reinterpret_cast<u64 *>(out), st);
+}
+
+// Roots of several depths: one pg_perft_batch per depth present (each level
+// loop runs one remaining depth for all its boards), shallowest first.
+int pg_perft_batch_mixed(pg_ctx *ctx, const pg_pos48 *pos, const uint32_t *depths, uint32_t n, int slider, int dedup,
+                         int blocks_per_cu, pg_tt *tt, uint64_t *out, pg_batch_stats *st) {
+    if (!ctx || (n && (!pos || !depths || !out))) return PG_EINVAL;
+    for (uint32_t i = 0; i < n; i++)
+        if (depths[i] >= TT_MAX_DEPTH) {
+            set_error("root %u: depth %u is beyond the largest supported (%d)", i, depths[i], TT_MAX_DEPTH - 1);
+            return PG_EINVAL;
+        }
+    const auto t0 = std::chrono::steady_clock::now();
+    if (st) memset(st, 0, sizeof(*st));
+    std::vector<uint32_t> order(n);
+    std::iota(order.begin(), order.end(), 0u);
+    std::stable_sort(order.begin(), order.end(), [&](uint32_t a, uint32_t b) { return depths[a] < depths[b]; });
+    std::vector<pg_pos48> group;
+    std::vector<uint64_t> res;
+    for (uint32_t a = 0; a < n;) {
+        const uint32_t d = depths[order[a]];
+        uint32_t b = a;
+        while (b < n && depths[order[b]] == d) b++;
+        group.resize(b - a);
+        res.resize(b - a);
+        for (uint32_t k = a; k < b; k++) group[k - a] = pos[order[k]];
+        pg_batch_stats gs{};
+        const int rc = pg_perft_batch(ctx, group.data(), b - a, d, slider, dedup, blocks_per_cu, tt, res.data(), &gs);
+        if (rc) {
+            const std::string why = pg_last_error();  // set_error writes the buffer it returns
+            set_error("%s (the roots of depth %u)", why.c_str(), d);
+            return rc;
+        }
+        for (uint32_t k = a; k < b; k++) out[order[k]] = res[k - a];
+        if (st) {
+            st->graph_ms += gs.graph_ms;
+            st->launches += gs.launches;
+            st->splits += gs.splits;
+            st->leaf_items += gs.leaf_items;
+            st->graphs_cached = gs.graphs_cached;
+            st->peak_bytes = std::max(st->peak_bytes, gs.peak_bytes);
+        }
+        a = b;
+    }
+    if (st) st->total_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+    return PG_OK;
 }
 
 }  // extern "C"

@@ -40,6 +40,39 @@ def test_batch_mixed_sides_and_duplicates(dev, pg):
     assert [int(x) for x in got] == [items[i]["perft"][4] for i in idx]
 
 
+@pytest.mark.parametrize("with_tt", [False, True])
+def test_batch_of_mixed_depths(dev, pg, with_tt):
+    """pg_perft_batch_mixed: a depth per root (0-4, shuffled, the same board
+    at several depths), against the reference's values; then the c2
+    positions at depth 6 in the same call as random ones at depth 3."""
+    arr, items = random_set(200)
+    rng = np.random.default_rng(7)
+    depths = rng.integers(0, 5, size=len(arr)).astype(np.uint32)
+    idx = np.concatenate([np.arange(len(arr)), [5, 5, 5]])
+    depths = np.concatenate([depths, [0, 2, 4]]).astype(np.uint32)
+    tt = pg.TranspositionTable(dev, 1 << 30) if with_tt else None
+    try:
+        print("mixed-depth batch tt", with_tt, flush=True)
+        got, st = dev.perft_batch_mixed(arr[idx], depths, 0, True, tt=tt)
+        assert [int(x) for x in got] == [items[i]["perft"][d] for i, d in zip(idx, depths)]
+        assert st["launches"] == len(set(depths.tolist()) - {0})  # depth 0 needs no launch
+        c2 = golden("c2.json")["positions"]
+        both = np.concatenate([np.array([x["words"] for x in c2], np.uint64), arr[:32]])
+        d2 = np.array([6] * len(c2) + [3] * 32, np.uint32)
+        got, _ = dev.perft_batch_mixed(both, d2, 0, True, tt=tt)
+        assert [int(x) for x in got] == [x["perft"][6] for x in c2] + [it["perft"][3] for it in items[:32]]
+    finally:
+        if tt:
+            tt.close()
+    got, _ = dev.perft_batch_mixed(arr[:0], np.zeros(0, np.uint32))
+    assert len(got) == 0
+    with pytest.raises(pg.PerftError) as e:
+        dev.perft_batch_mixed(arr[:2], np.array([3, 16], np.uint32))
+    assert e.value.code == pg.PG_EINVAL
+    with pytest.raises(ValueError):
+        dev.perft_batch_mixed(arr[:2], [3])
+
+
 def test_graph_replays_with_eager_work_between(dev, pg):
     c2 = golden("c2.json")["positions"]
     arr = np.array([x["words"] for x in c2], np.uint64)
