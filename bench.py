@@ -146,20 +146,55 @@ def leg_tt(pg, sizes=(16, 64), depths=(9, 10)):
             "arena_gib": arena / (1 << 30), "runs": out, "exact": True}
 
 
+TT_SHALLOW_MAX = 4          # remaining depths with 16-byte entries; deeper ones take 32 bytes
+LINE = 64                   # bytes a random access moves when rounded to whole lines
+
+
+def tt_access_bytes(nodes, ctr):
+    """The table's own HBM traffic in one TT search, algorithmic and rounded
+    to one 64-byte line per random access.  nodes[r]: boards made at
+    remaining depth r; ctr[r] = (probes, hits, stores, duplicates) of the
+    search.  Per probe and per store one entry (16 B at r <= 4, 32 B deeper);
+    per board of a level with a duplicate set (r >= 2) that the table did not
+    answer, one 4-byte set slot read and written; per duplicate its owner's
+    16-byte key (compared) and 8-byte perft (copied).  The streams every
+    level has anyway (boards, keys, counts, per-board perfts) are not
+    included."""
+    alg = rnd = 0
+    acc = {"probes": 0, "stores": 0, "set_inserts": 0, "duplicates": 0}
+    for r, (probes, hits, stores, dups) in ctr.items():
+        e = 16 if r <= TT_SHALLOW_MAX else 32
+        inserts = max(0, nodes.get(r, 0) - hits) if r >= 2 else 0
+        alg += e * (probes + stores) + 8 * inserts + 24 * dups
+        rnd += LINE * (probes + stores) + 2 * LINE * inserts + 2 * LINE * dups
+        acc["probes"] += probes
+        acc["stores"] += stores
+        acc["set_inserts"] += inserts
+        acc["duplicates"] += dups
+    acc.update({"algorithmic_gb": alg / 1e9, "line_rounded_gb": rnd / 1e9})
+    return acc
+
+
 def tt_runs(pg, W, dev, gb, depths):
     """TT perft of the start position at each depth on one new table of gb GB."""
+    import numpy as np
     out = []
     with pg.TranspositionTable(dev, gb << 30) as tt:
         for d in depths:
+            c0 = tt.counters().astype(np.int64)
             t = time.perf_counter()
             n = tt.perft(pg.START_FEN, d, 0)
             dt = time.perf_counter() - t
             if n != W.CPW_COUNTS["start"][d]:
                 raise SystemExit("MISMATCH TT perft(%d)" % d)
             st = dev.last_stats()
+            c = tt.counters().astype(np.int64) - c0
+            nodes = {d - p: st["nodes"][p] for p in range(1, d)}
+            ctr = {r: tuple(int(x) for x in c[r]) for r in range(2, d + 1)}
             out.append({"tt_gb": gb, "depth": d, "perft": n, "s": dt, "effective_leaves_per_s": n / dt,
                         "peak_arena_gb": st["peak_arena"] / 1e9, "boards_per_ply": st["nodes"][1:d],
-                        "leaf_items": st["leaf_items"], "splits": st["splits"]})
+                        "leaf_items": st["leaf_items"], "splits": st["splits"],
+                        "table_access": tt_access_bytes(nodes, ctr)})
             log("tt %d GB d%d %.3f s" % (gb, d, dt))
         c = tt.counters()
         out[-1]["counters_by_remaining_depth"] = {
