@@ -380,6 +380,10 @@ __device__ __forceinline__ void leaf_tiles(const Pos *__restrict__ boards, const
     }
 }
 
+// Occupancy targets: four waves per SIMD for Kogge-Stone, five for the magic
+// variants (their table lookups wait on L2, more waves hide it).  Some magic
+// instantiations do not fit five and the compiler says so (-Wpass-failed);
+// they end at four, and asking them for four measured no faster.
 template <int V, bool PER>
 __global__ __launch_bounds__(LEAF_WG) __attribute__((amdgpu_waves_per_eu(V == SL_KOGGE ? 4 : 5, 8))) void k_leaf_fused(
     const Pos *__restrict__ boards, const u32 *__restrict__ cnt, u64 n, u64 *acc, u64 *items, u64 *__restrict__ per,
