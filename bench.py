@@ -122,10 +122,12 @@ def leg_batch(pg, dev):
             "ms": dt * 1e3, "graph_ms": st["graph_ms"], "leaves_per_s": total / dt, "launches": st["launches"],
             "autotuned": {"slider": pg.SLIDER_NAMES[cfg["slider"]], "dedup": cfg["dedup"],
                           "blocks_per_cu": cfg["blocks_per_cu"], "split_depth": cfg["split"],
-                          "grid": "sized to the batch" if cfg["grid"] < 0 else "whole chip"},
+                          "grid": ("%d threads per position" % cfg["tpp"]) if cfg.get("tpp")
+                          else "sized to the batch" if cfg["grid"] < 0 else "whole chip"},
             "autotune_table": [{"slider": pg.SLIDER_NAMES[c["slider"]], "dedup": c["dedup"],
                                 "blocks_per_cu": c["blocks_per_cu"], "split_depth": c["split"],
-                                "grid": "batch" if c["grid"] < 0 else "chip", "threads_per_position": tpp,
+                                "grid": "tpp" if c.get("tpp") else "batch" if c["grid"] < 0 else "chip",
+                                "threads_per_position": tpp,
                                 "ms_on_512_sample": t_ * 1e3} for c, t_, tpp in table],
             "one_position_per_launch": {"ms": loop * 1e3, "leaves_per_s": total / loop},
             "speedup_vs_plain_loop": loop / dt, "exact": True}

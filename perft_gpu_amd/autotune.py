@@ -17,7 +17,10 @@ positions, every combination of
   chunk          positions per graph launch (None: the whole batch); the
                  default space holds only None (one launch per batch), and
                  callers may pass candidates
-and returns the fastest.  Every timed run is checked against the first
+and then, on the fastest of those, every value of
+  tpp            threads per root position: the grid sized so that this many
+                 threads share each root (work-groups = tpp x roots / 256)
+and returns the fastest of all.  Every timed run is checked against the first
 configuration's counts, so a configuration that changed a result could never
 win.  The reference has no autotuner; its choices are compile-time switches
 (switches.h) and a fixed launch depth.
@@ -35,7 +38,13 @@ DEFAULT_SPACE = {
     "split": [0, 1],
     "grid": [-1, 0],
     "chunk": [None],
+    "tpp": [64, 256, 1024, 4096],
 }
+
+
+def grid_for_tpp(tpp, roots):
+    """Work-groups of 256 threads for tpp threads per root."""
+    return max(1, -(-tpp * roots // 256))
 
 
 def work_groups(dev_cus, n, depth, grid, blocks_per_cu):
@@ -64,7 +73,7 @@ def run(dev, arr, depth, cfg):
     work, owner = (children_host(arr, split) if split else (arr, np.arange(len(arr))))
     d = depth - split
     chunk = cfg.get("chunk") or len(work)
-    dev.set_batch_grid(cfg.get("grid", -1))
+    dev.set_batch_grid(grid_for_tpp(cfg["tpp"], len(arr)) if cfg.get("tpp") else cfg.get("grid", -1))
     try:
         parts = []
         for i in range(0, len(work), chunk):
@@ -87,6 +96,21 @@ def tune(dev, positions, depth, space=None, sample=512, seed=1, repeats=1, log=N
         arr = arr[np.sort(rng.choice(len(arr), sample, replace=False))]
     ref = None
     table = []
+
+    def timed(cfg):
+        nonlocal ref
+        run(dev, arr, depth, cfg)      # graph capture + warm caches
+        best = float("inf")
+        for _ in range(repeats):
+            t = time.perf_counter()
+            got = run(dev, arr, depth, cfg)
+            best = min(best, time.perf_counter() - t)
+        if ref is None:
+            ref = got
+        elif not np.array_equal(ref, got):
+            raise RuntimeError("autotune: counts differ for %s" % cfg)
+        return best
+
     for s in space["slider"]:
         for dd in space["dedup"]:
             for b in space["blocks_per_cu"]:
@@ -94,16 +118,7 @@ def tune(dev, positions, depth, space=None, sample=512, seed=1, repeats=1, log=N
                     for g in space["grid"]:
                         for ch in space["chunk"]:
                             cfg = {"slider": s, "dedup": dd, "blocks_per_cu": b, "split": sp, "grid": g, "chunk": ch}
-                            run(dev, arr, depth, cfg)      # graph capture + warm caches
-                            best = float("inf")
-                            for _ in range(repeats):
-                                t = time.perf_counter()
-                                got = run(dev, arr, depth, cfg)
-                                best = min(best, time.perf_counter() - t)
-                            if ref is None:
-                                ref = got
-                            elif not np.array_equal(ref, got):
-                                raise RuntimeError("autotune: counts differ for %s" % cfg)
+                            best = timed(cfg)
                             n_work = len(arr) * (30 ** sp)
                             tpp = work_groups(cus, n_work, depth - sp, g, b) * 256 / max(1, len(arr))
                             table.append((cfg, best, tpp))
@@ -111,5 +126,14 @@ def tune(dev, positions, depth, space=None, sample=512, seed=1, repeats=1, log=N
                                 log("autotune %s dedup=%d bpc=%d split=%d grid=%s chunk=%s: %.2f ms"
                                     % (SLIDER_NAMES[s], dd, b, sp, "batch" if g < 0 else ("chip" if g == 0 else g),
                                        ch, best * 1e3))
+    # threads per root position, on the fastest configuration so far
+    base = min(table, key=lambda x: x[1])[0]
+    for tpp in space.get("tpp") or []:
+        cfg = dict(base, tpp=tpp)
+        best = timed(cfg)
+        table.append((cfg, best, float(grid_for_tpp(tpp, len(arr)) * 256) / max(1, len(arr))))
+        if log:
+            log("autotune %s dedup=%d bpc=%d split=%d %d threads per position: %.2f ms"
+                % (SLIDER_NAMES[base["slider"]], base["dedup"], base["blocks_per_cu"], base["split"], tpp, best * 1e3))
     best_cfg = min(table, key=lambda x: x[1])[0]
     return best_cfg, table

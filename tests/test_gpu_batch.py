@@ -146,7 +146,12 @@ def test_split_depth_and_autotuner(dev, pg):
             assert np.array_equal(autotune.run(dev, arr, 4, cfg), want), cfg
     cfg = {"slider": 2, "dedup": False, "blocks_per_cu": 4, "split": 1, "grid": 0, "chunk": 32}
     assert np.array_equal(autotune.run(dev, arr, 4, cfg), want)
-    space = {"slider": [0, 2], "dedup": [True], "blocks_per_cu": [8], "split": [0, 1], "grid": [-1, 0]}
+    for tpp in (16, 64, 4096):   # threads per root: 16 puts the 128 roots on 8 work-groups
+        cfg = {"slider": 0, "dedup": True, "blocks_per_cu": 8, "split": 0, "grid": -1, "chunk": None, "tpp": tpp}
+        assert np.array_equal(autotune.run(dev, arr, 4, cfg), want), cfg
+    space = {"slider": [0, 2], "dedup": [True], "blocks_per_cu": [8], "split": [0, 1], "grid": [-1, 0],
+             "tpp": [256, 1024]}
     best, table = autotune.tune(dev, arr, 4, space=space, sample=64)
-    assert len(table) == 8 and best in [c for c, _, _ in table]
+    assert len(table) == 8 + 2 and best in [c for c, _, _ in table]
+    assert [c.get("tpp") for c, _, _ in table[8:]] == [256, 1024]
     assert np.array_equal(autotune.run(dev, arr, 4, best), want)
