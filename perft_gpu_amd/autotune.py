@@ -20,6 +20,8 @@ positions, every combination of
 and then, on the fastest of those, every value of
   tpp            threads per root position: the grid sized so that this many
                  threads share each root (work-groups = tpp x roots / 256)
+  split_deeper   split depths beyond the first search's (two host plies: a
+                 root's ~1000 grandchildren each become a batch root)
 and returns the fastest of all.  Every timed run is checked against the first
 configuration's counts, so a configuration that changed a result could never
 win.  The reference has no autotuner; its choices are compile-time switches
@@ -39,6 +41,7 @@ DEFAULT_SPACE = {
     "grid": [-1, 0],
     "chunk": [None],
     "tpp": [64, 256, 1024, 4096],
+    "split_deeper": [2],
 }
 
 
@@ -126,8 +129,18 @@ def tune(dev, positions, depth, space=None, sample=512, seed=1, repeats=1, log=N
                                 log("autotune %s dedup=%d bpc=%d split=%d grid=%s chunk=%s: %.2f ms"
                                     % (SLIDER_NAMES[s], dd, b, sp, "batch" if g < 0 else ("chip" if g == 0 else g),
                                        ch, best * 1e3))
-    # threads per root position, on the fastest configuration so far
+    # threads per root position and deeper splits, on the fastest configuration so far
     base = min(table, key=lambda x: x[1])[0]
+    for sp in space.get("split_deeper") or []:
+        cfg = dict(base, split=sp)
+        best = timed(cfg)
+        n_work = len(arr) * (30 ** sp)
+        table.append((cfg, best, work_groups(cus, n_work, depth - sp, base["grid"], base["blocks_per_cu"]) * 256
+                      / max(1, len(arr))))
+        if log:
+            log("autotune %s dedup=%d bpc=%d split=%d grid=%s: %.2f ms"
+                % (SLIDER_NAMES[base["slider"]], base["dedup"], base["blocks_per_cu"], sp,
+                   "batch" if base["grid"] < 0 else "chip", best * 1e3))
     for tpp in space.get("tpp") or []:
         cfg = dict(base, tpp=tpp)
         best = timed(cfg)

@@ -150,8 +150,8 @@ def test_split_depth_and_autotuner(dev, pg):
         cfg = {"slider": 0, "dedup": True, "blocks_per_cu": 8, "split": 0, "grid": -1, "chunk": None, "tpp": tpp}
         assert np.array_equal(autotune.run(dev, arr, 4, cfg), want), cfg
     space = {"slider": [0, 2], "dedup": [True], "blocks_per_cu": [8], "split": [0, 1], "grid": [-1, 0],
-             "tpp": [256, 1024]}
+             "tpp": [256, 1024], "split_deeper": [2]}
     best, table = autotune.tune(dev, arr, 4, space=space, sample=64)
-    assert len(table) == 8 + 2 and best in [c for c, _, _ in table]
-    assert [c.get("tpp") for c, _, _ in table[8:]] == [256, 1024]
+    assert len(table) == 8 + 1 + 2 and best in [c for c, _, _ in table]
+    assert table[8][0]["split"] == 2 and [c.get("tpp") for c, _, _ in table[9:]] == [256, 1024]
     assert np.array_equal(autotune.run(dev, arr, 4, best), want)
