@@ -390,6 +390,7 @@ def leg_serve(pg, dev):
         out = {"rate_per_s": rate if rate else "burst", "queries": len(idx), "elapsed_s": r["elapsed"],
                "leaves_per_s": leaves / r["elapsed"], "batches": r["batches"], "mean_batch": r["mean_batch"],
                "max_busy_slots": r["max_busy_slots"], "max_long_slots": r["max_long_slots"],
+               "device_ms": r["device_ms"], "batches_in_flight": r["batches_in_flight"],
                "plain_leaves_per_s": leaves / float(took[idx].sum())}
         out.update(latency_summary(r["latency"]))
         lat, dq, qd = np.asarray(r["latency"]), depths[idx], np.asarray(r["queued"]) * 1e3
@@ -397,8 +398,8 @@ def leg_serve(pg, dev):
                                         queued_p99_ms=float(np.percentile(qd[dq == d], 99)))
                            for d in sorted(set(int(x) for x in dq))}
         out["submit_ms_max"] = r["stats"]["submit_ms_max"]
-        log("serve rate %s: p50 %.2f ms p99 %.2f ms %.1f G leaves/s" % (rate, out["p50_ms"], out["p99_ms"],
-                                                                      out["leaves_per_s"] / 1e9))
+        log("serve rate %s: p50 %.2f ms p99 %.2f ms %.1f G leaves/s, %.2f batches in flight"
+            % (rate, out["p50_ms"], out["p99_ms"], out["leaves_per_s"] / 1e9, out["batches_in_flight"]))
         return out
 
     rates, first = [], 0

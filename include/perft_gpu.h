@@ -358,6 +358,9 @@ typedef struct pg_serve_stats {
     double est_bytes_per_query[16];          /* current admission estimate per depth */
     double submit_ms_max, submit_ms_total;   /* host time spent launching batches */
     uint64_t graphs;                         /* graphs instantiated by the slots so far */
+    double device_ms;                        /* device time of the finished batches, summed (HIP
+                                                events around each batch's graph): over elapsed_ms,
+                                                the mean number of batches running at once */
 } pg_serve_stats;
 typedef struct pg_serve pg_serve;
 int pg_serve_create(pg_ctx *ctx, pg_tt *tt /* may be NULL */, const pg_serve_config *cfg, pg_serve **out);

@@ -702,7 +702,9 @@ int pg::server_submit(Server &srv, uint32_t slot, const pg_pos48 *pos, uint32_t 
     h.bias = e.bias;
     h.lv[0] = L0;
     HIP_TRY(hipMemcpyAsync(sl.dPlan, &h, sizeof(Plan), hipMemcpyHostToDevice, sl.stream));
+    HIP_TRY(hipEventRecord(sl.ev0, sl.stream));  // the batch's device time (pg_batch_stats.graph_ms)
     HIP_TRY(hipGraphLaunch(exec, sl.stream));
+    HIP_TRY(hipEventRecord(sl.ev1, sl.stream));
     HIP_TRY(hipMemcpyAsync(&h, sl.dPlan, sizeof(Plan), hipMemcpyDeviceToHost, sl.stream));
     HIP_TRY(hipMemcpyAsync(ss.hOut, L0.perft, n * 8, hipMemcpyDeviceToHost, sl.stream));
     HIP_TRY(hipEventRecord(ss.done, sl.stream));
@@ -743,7 +745,9 @@ int pg_server_poll(pg_server *s, uint32_t slot, uint64_t *out, pg_batch_stats *s
         return PG_EINVAL;
     }
     u64 items = h.leafItems, launches = 1, splits = 0, peak = h.peak;
-    double ms = 0;
+    float first = 0;
+    (void)hipEventElapsedTime(&first, sl.ev0, sl.ev1);  // read before a split reuses the events
+    double ms = first;
     if (h.overflow) {
         GraphShape sh{(int)ss.depth, ss.v, 1, ss.dedup ? 1 : 0, e.bpc, s->tt, 0, ss.wgCap,
                       grid_limit(e, ss.n, (int)ss.depth)};
@@ -760,6 +764,7 @@ int pg_server_poll(pg_server *s, uint32_t slot, uint64_t *out, pg_batch_stats *s
     if (st) {
         memset(st, 0, sizeof(*st));
         st->total_ms = host_ms() - ss.t0;
+        st->graph_ms = ms;
         st->launches = launches;
         st->splits = splits;
         st->leaf_items = items;

@@ -245,6 +245,7 @@ int pg_serve_poll(pg_serve *s, pg_serve_result *out, uint32_t max, uint32_t *n) 
         sl.graphs = std::max(sl.graphs, bs.graphs_cached);
         s->st.launches += bs.launches;
         s->st.splits += bs.splits;
+        s->st.device_ms += bs.graph_ms;
         s->lastDone = t;
         if (r == 1 && !sl.ids.empty() && bs.peak_bytes) {
             // learned region bytes per query of this depth, with headroom

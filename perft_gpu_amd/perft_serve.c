@@ -114,9 +114,9 @@ int main(int argc, char **argv) {
         leaves += answer[i];
     }
     printf("served %u queries at %.0f/s in %.3f s: %.1f G leaves/s, %llu batches, %u slots busy at most, "
-           "p50 %.3f ms, p99 %.3f ms, mismatches %llu\n",
-           n, rate, elapsed, leaves / elapsed / 1e9, (unsigned long long)st.batches, st.max_busy_slots, st.p50_ms,
-           st.p99_ms, (unsigned long long)bad);
+           "%.2f batches on the device on average, p50 %.3f ms, p99 %.3f ms, mismatches %llu\n",
+           n, rate, elapsed, leaves / elapsed / 1e9, (unsigned long long)st.batches, st.max_busy_slots,
+           st.elapsed_ms > 0 ? st.device_ms / st.elapsed_ms : 0.0, st.p50_ms, st.p99_ms, (unsigned long long)bad);
     pg_serve_destroy(srv);
     if (tt) pg_tt_destroy(tt);
     pg_shutdown(ctx);

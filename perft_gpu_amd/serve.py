@@ -38,7 +38,7 @@ class ServeStats(ctypes.Structure):
                 ("p50_ms", ctypes.c_double), ("p99_ms", ctypes.c_double), ("mean_ms", ctypes.c_double),
                 ("max_ms", ctypes.c_double), ("elapsed_ms", ctypes.c_double),
                 ("est_bytes_per_query", ctypes.c_double * 16), ("submit_ms_max", ctypes.c_double),
-                ("submit_ms_total", ctypes.c_double), ("graphs", u64)]
+                ("submit_ms_total", ctypes.c_double), ("graphs", u64), ("device_ms", ctypes.c_double)]
 
     def as_dict(self):
         d = {n: getattr(self, n) for n, _ in self._fields_ if n != "est_bytes_per_query"}
@@ -161,8 +161,9 @@ def serve(loop, positions, depths, arrivals):
     arrival (the loop's own submit-to-done latency plus any delay before the
     submit); `queued` is the part of it before the query's batch was
     launched.  Returns results, latencies (s), elapsed time, this stream's
-    batches and the loop's stats (max_busy_slots and max_long_slots are
-    maxima over the loop's lifetime)."""
+    batches, their summed device time and its ratio to the elapsed time (the
+    mean number of batches running at once), and the loop's stats
+    (max_busy_slots and max_long_slots are maxima over the loop's lifetime)."""
     positions = positions_array(positions)
     n = len(positions)
     results = np.zeros(n, dtype=np.uint64)
@@ -193,7 +194,9 @@ def serve(loop, positions, depths, arrivals):
     st = loop.stats()
     # the loop's counters run over its lifetime: this stream's share
     batches = st["batches"] - st0["batches"]
+    device_ms = st["device_ms"] - st0["device_ms"]
     return {"results": results, "latency": latency, "queued": queued, "elapsed": elapsed, "batches": batches,
+            "device_ms": device_ms, "batches_in_flight": device_ms / max(elapsed * 1e3, 1e-9),
             "mean_batch": (st["completed"] - st0["completed"]) / max(1, batches), "max_busy_slots": st["max_busy_slots"],
             "max_long_slots": st["max_long_slots"], "splits": st["splits"], "stats": st}
 

@@ -32,6 +32,9 @@ def test_serve_mixed_stream_exact(dev, pg):
     assert st["max_busy_slots"] >= 2                     # batches overlapped on the device
     assert st["max_long_slots"] <= 2
     assert 0 < st["p50_ms"] <= st["p99_ms"] <= st["max_ms"]
+    # device time of the batches: positive, and at most one batch per slot at a time
+    assert 0 < out["device_ms"] <= 4 * out["elapsed"] * 1e3 * 1.05
+    assert out["batches_in_flight"] > 0
     # the same query alone, on the plain path
     for i in range(0, n, 37):
         assert dev.perft(pg.pos_from_words(pos[i]), int(depths[i]), 0) == want[i]
@@ -92,7 +95,7 @@ def test_served_batch_overflow_finishes_in_halves(dev, pg):
             r = srv.poll(0)
         counts, st, _ = r
     assert [int(x) for x in counts] == [x["perft"][5] for x in g]
-    assert st["splits"] > 0
+    assert st["splits"] > 0 and st["graph_ms"] > 0
 
 
 def test_server_rejects_misuse(dev, pg):
