@@ -268,10 +268,10 @@ typedef struct pg_batch_stats {
  * pos[i]. */
 int pg_perft_batch(pg_ctx *ctx, const pg_pos48 *pos, uint32_t n, uint32_t depth, int slider, int dedup,
                    int blocks_per_cu, pg_tt *tt, uint64_t *out, pg_batch_stats *stats);
-/* Roots with a depth each (the same reference counterpart): one
- * pg_perft_batch per depth present, shallowest first; out[i] = perft of
- * pos[i] to depths[i].  stats sums the calls (peak_bytes: their maximum,
- * total_ms: the whole call). */
+/* Batched perft with a depth per root (launcher.h:966-1179's sibling
+ * batches, as pg_perft_batch): one pg_perft_batch per depth present,
+ * shallowest first; out[i] = perft of pos[i] to depths[i].  stats sums the
+ * calls (peak_bytes: their maximum, total_ms: the whole call). */
 int pg_perft_batch_mixed(pg_ctx *ctx, const pg_pos48 *pos, const uint32_t *depths, uint32_t n, int slider, int dedup,
                          int blocks_per_cu, pg_tt *tt, uint64_t *out, pg_batch_stats *stats);
 /* Work-groups per kernel of later pg_perft_batch calls (the threads that
