@@ -82,6 +82,7 @@ struct Engine {
     bool dedup = true;
     int tt_bfs(int v, TTable &tt, Pos *B, Key128 *K, u64 n, u32 *cnt, int r, u64 *P, int ply);
     int tt_split(int v, TTable &tt, Pos *B, Key128 *K, u64 n, u32 *cnt, int r, u64 *P, int ply);
+    int tt_last(int v, TTable &tt, Pos *B, Key128 *K, u64 n, u32 *cnt, u64 *P, int ply);
     int perft_tt(TTable &tt, const Pos &root, u32 depth, int v, u64 *out);
 };
 

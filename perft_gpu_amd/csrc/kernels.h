@@ -303,13 +303,13 @@ enum : u8 { ST_LIVE = 0, ST_HIT = 1, ST_DUP = 2 };
 // MOVES: the n boards are not stored: board j is boards[par[j]] after move
 // mv[j] (the children of the level before, as the generate kernel lists
 // them); the work-group makes each one and counts its moves itself, so the
-// level one above the leaf never goes to HBM.  Sums only (PER = false).
+// level one above the leaf never goes to HBM.  With idx as well, the boards
+// made are those listed (j = idx[t]), and with PER their sums go to per[j].
 template <int V, bool PER, bool MOVES = false>
 __device__ __forceinline__ void leaf_tiles(const Pos *__restrict__ boards, const u32 *__restrict__ cnt, u64 n, u64 *acc,
                                            u64 *items, u64 *__restrict__ per, const u8 *__restrict__ state, u64 base,
                                            u32 *err, u32 *ovf, const Tables &t, const u32 *__restrict__ idx = nullptr,
                                            const u32 *__restrict__ par = nullptr, const u16 *__restrict__ mv = nullptr) {
-    static_assert(!(MOVES && PER), "a leaf over moves only sums");
     static_assert(V != SL_BYTE, "the byte-lookup tables leave no LDS for the move window");
     __shared__ Pos sb[LEAF_WG];
     constexpr int LEAF_CAP = leaf_cap<PER>();
@@ -327,8 +327,8 @@ __device__ __forceinline__ void leaf_tiles(const Pos *__restrict__ boards, const
         u32 c = 0;
         if constexpr (MOVES) {
             if (live) {
-                Pos b = boards[par[t0]];
-                make_any(b, mv[t0], tl);
+                Pos b = boards[par[i]];
+                make_any(b, mv[i], tl);
                 c = count_any<V>(b, tl);
                 sb[tid] = b;
             }
@@ -399,35 +399,51 @@ __global__ __launch_bounds__(LEAF_WG) __attribute__((amdgpu_waves_per_eu(V == SL
     leaf_tiles<V, false, true>(gp, nullptr, n, acc, items, nullptr, nullptr, 0, err, ovf, t, nullptr, par, mv);
 }
 
+// per-child sums over the children of `gp` listed by idx (see leaf_tiles,
+// MOVES with idx): perft(2) of each listed child into per[]
+template <int V>
+__global__ __launch_bounds__(LEAF_WG) __attribute__((amdgpu_waves_per_eu(V == SL_KOGGE ? 4 : 5, 8))) void k_leaf_moves_per(
+    const Pos *__restrict__ gp, const u32 *__restrict__ par, const u16 *__restrict__ mv, const u32 *__restrict__ idx,
+    u64 n, u64 *items, u64 *__restrict__ per, u32 *err, u32 *ovf, Tables t) {
+    leaf_tiles<V, true, true>(gp, nullptr, n, nullptr, items, per, nullptr, 0, err, ovf, t, idx, par, mv);
+}
+
 // ---------------------------------------------------------------- live list
 // The boards of a leaf level that have moves, in index order: a level whose
 // other boards were resolved from the table or collapsed into an equal board
 // (count 0) then fills the fused leaf's tiles with boards that have work.
 // Each work-group takes LIVE_CHUNK consecutive boards: k_live_count counts
 // its live boards, the counts are scanned, and k_live_write lists them at
-// the chunk's offset.
+// the chunk's offset.  With state given, a live board is one whose state is
+// ST_LIVE (a level whose boards were not written, so not counted either).
 constexpr int LIVE_WG = 256, LIVE_PER_THREAD = 16, LIVE_CHUNK = LIVE_WG * LIVE_PER_THREAD;
 
-static __global__ __launch_bounds__(LIVE_WG) void k_live_count(const u32 *__restrict__ cnt, u64 n, u32 *__restrict__ chunkCount) {
+__device__ __forceinline__ bool live_at(const u32 *__restrict__ cnt, const u8 *__restrict__ state, u64 i) {
+    return state ? state[i] == ST_LIVE : cnt[i] != 0;
+}
+
+static __global__ __launch_bounds__(LIVE_WG) void k_live_count(const u32 *__restrict__ cnt, const u8 *__restrict__ state,
+                                                              u64 n, u32 *__restrict__ chunkCount) {
     const u64 base = (u64)blockIdx.x * LIVE_CHUNK;
     u32 c = 0;
 #pragma unroll
     for (int k = 0; k < LIVE_PER_THREAD; k++) {
         const u64 i = base + (u64)k * LIVE_WG + threadIdx.x;
-        c += i < n && cnt[i] != 0;
+        c += i < n && live_at(cnt, state, i);
     }
     u64 total;
     (void)block_excl_scan<LIVE_WG>(c, &total);
     if (threadIdx.x == 0) chunkCount[blockIdx.x] = (u32)total;
 }
 
-static __global__ __launch_bounds__(LIVE_WG) void k_live_write(const u32 *__restrict__ cnt, u64 n,
-                                                              const u64 *__restrict__ chunkOff, u32 *__restrict__ idx) {
+static __global__ __launch_bounds__(LIVE_WG) void k_live_write(const u32 *__restrict__ cnt, const u8 *__restrict__ state,
+                                                              u64 n, const u64 *__restrict__ chunkOff,
+                                                              u32 *__restrict__ idx) {
     const u64 base = (u64)blockIdx.x * LIVE_CHUNK;
     u64 at = chunkOff[blockIdx.x];
     for (int k = 0; k < LIVE_PER_THREAD; k++) {
         const u64 i = base + (u64)k * LIVE_WG + threadIdx.x;
-        const bool f = i < n && cnt[i] != 0;
+        const bool f = i < n && live_at(cnt, state, i);
         u64 total;
         const u64 pos = block_excl_scan<LIVE_WG>(f ? 1 : 0, &total);
         if (f) idx[at + pos] = (u32)i;

@@ -128,6 +128,37 @@ __global__ __launch_bounds__(wg_size<V>()) void k_expand_tt(const Pos *__restric
     }
 }
 
+// The last materialized level of a table search (children of remaining
+// depth 2): each child is made in registers only, for its key; the board and
+// its move count are not written.  Equal keys are collapsed on the keys
+// alone, and the fused leaf makes the children that remain from their parent
+// and move (k_leaf_moves_per), so a collapsed child costs no move count and
+// no 48-byte board.
+template <int V>
+__global__ __launch_bounds__(wg_size<V>()) void k_expand_keys(const Pos *__restrict__ parents,
+                                                     const Key128 *__restrict__ pkeys, const u32 *__restrict__ pidx,
+                                                     const u16 *__restrict__ moves, u64 m, Key128 *__restrict__ ckeys,
+                                                     u64 *__restrict__ cperft, u8 *__restrict__ state, TTView tt,
+                                                     int rc, TTCounters *ctr, Tables t) {
+    stage_tables<V>(t);
+    PG_GRID_LOOP(i, m) {
+        const u32 pi = pidx[i];
+        Pos p = parents[pi];
+        Key128 h = pkeys[pi];
+        make_any_hash(p, moves[i], h, t);
+        u64 v = 0;
+        const bool probe = rc >= 2 && tt.base[rc];
+        const bool hit = probe && tt_probe(tt, rc, h, &v);
+        if (ctr) {
+            ctr_count(ctr, rc, 0, probe);
+            ctr_count(ctr, rc, 1, hit);
+        }
+        ckeys[i] = h;
+        cperft[i] = v;
+        state[i] = hit ? ST_HIT : ST_LIVE;
+    }
+}
+
 PG_HD bool same_board(const Pos &a, const Pos &b) {
     const u64 clk = ~(0x7FULL << 56);  // the half-move clock is not part of a position
     return a.white == b.white && (a.pawns & clk) == (b.pawns & clk) && a.knights == b.knights &&
@@ -167,8 +198,8 @@ __device__ __forceinline__ void dedup_finish(u64 i, bool live, u64 s, u32 cur, P
     if (ctr) ctr_count(ctr, rc, 3, isDup);
     if (isDup) {
         state[i] = ST_DUP;
-        cnt[i] = 0;
-        children[i].white = 0;
+        if (cnt) cnt[i] = 0;  // a level whose boards and counts were not written (k_expand_keys) has neither
+        if (children) children[i].white = 0;
     }
 }
 

@@ -32,6 +32,19 @@ static void launch_leaf_per(Engine &c, const Pos *B, const u32 *cnt, u64 n, u64 
         k_leaf_fused<V, true><<<c.leaf_grid(n), LEAF_WG, 0, c.stream>>>(B, cnt, n, c.dAcc, c.dItems, P, c.dErr, c.dOvf,
                                                                         c.dt, idx);
 }
+template <int V> static void launch_expand_keys(Engine &c, TTable &tt, const Pos *B, const Key128 *K, const u32 *par,
+                                                const u16 *mv, u64 m, Key128 *CK, u64 *CP, u8 *S, int rc) {
+    if constexpr (V != SL_BYTE) {
+        constexpr int W = wg_size<V>();
+        k_expand_keys<V><<<c.grid(m, W), W, 0, c.stream>>>(B, K, par, mv, m, CK, CP, S, tt.view, rc, tt.dCtr, c.dt);
+    }
+}
+template <int V>
+static void launch_leaf_moves_per(Engine &c, const Pos *B, const u32 *par, const u16 *mv, const u32 *idx, u64 n, u64 *P) {
+    if constexpr (V != SL_BYTE)
+        k_leaf_moves_per<V><<<c.leaf_grid(n), LEAF_WG, 0, c.stream>>>(B, par, mv, idx, n, c.dItems, P, c.dErr, c.dOvf,
+                                                                      c.dt);
+}
 template <int V> static void launch_gen(Engine &c, const Pos *b, u64 n, const u64 *off, u16 *mv, u32 *par) {
     constexpr int W = wg_size<V>();
     k_generate<V><<<c.grid(n, W), W, 0, c.stream>>>(b, n, off, mv, par, c.dErr, c.dt);
@@ -80,7 +93,7 @@ int Engine::tt_bfs(int v, TTable &tt, Pos *B, Key128 *K, u64 n, u32 *cnt, int r,
         u32 *idx = nullptr;
         u64 nLive = n;
         if (chunkOff) {
-            k_live_count<<<(u32)chunks, LIVE_WG, 0, stream>>>(cnt, n, chunkCount);
+            k_live_count<<<(u32)chunks, LIVE_WG, 0, stream>>>(cnt, nullptr, n, chunkCount);
             HIP_TRY(hipGetLastError());
             int rc = scan(chunkCount, chunks, chunkOff, &nLive);
             if (rc && rc != PG_ENOMEM) {
@@ -89,7 +102,7 @@ int Engine::tt_bfs(int v, TTable &tt, Pos *B, Key128 *K, u64 n, u32 *cnt, int r,
             }
             idx = rc == PG_OK ? (u32 *)alloc(nLive * sizeof(u32) + 4) : nullptr;
             if (idx) {
-                k_live_write<<<(u32)chunks, LIVE_WG, 0, stream>>>(cnt, n, chunkOff, idx);
+                k_live_write<<<(u32)chunks, LIVE_WG, 0, stream>>>(cnt, nullptr, n, chunkOff, idx);
                 HIP_TRY(hipGetLastError());
             } else {
                 nLive = n;
@@ -111,6 +124,7 @@ int Engine::tt_bfs(int v, TTable &tt, Pos *B, Key128 *K, u64 n, u32 *cnt, int r,
         stats.leaf_parents += n;
         return PG_OK;
     }
+    if (r == 3 && v != SL_BYTE) return tt_last(v, tt, B, K, n, cnt, P, ply);
     if (n > 0xFFFFFFFFULL) return tt_split(v, tt, B, K, n, cnt, r, P, ply);
     const size_t mark = top;
     u64 *off = (u64 *)alloc((n + 1) * sizeof(u64));
@@ -156,6 +170,83 @@ int Engine::tt_bfs(int v, TTable &tt, Pos *B, Key128 *K, u64 n, u32 *cnt, int r,
         top = keep;  // moves, parents and the set are no longer needed
         rc = tt_bfs(v, tt, C, CK, m, Ccnt, rcd, CP, ply + 1);
         if (rc) return rc;
+        k_resolve_store<<<grid(m, 256), 256, 0, stream>>>(CK, m, CP, S, D, tt.view, rcd, tt.dCtr);
+    }
+    k_segsum<<<grid(n, 256), 256, 0, stream>>>(B, n, off, CP, P, ply == 0 ? bias : 0, dOvf);
+    HIP_TRY(hipGetLastError());
+    top = mark;
+    return PG_OK;
+}
+
+// The boards of remaining depth 3 (B, n) and the children they expand to,
+// which are never written as boards: per child its key, perft, owner, state,
+// move and parent index (35 bytes, and 8 for the collapse set) instead of
+// the 81 of a written board with its count.  k_expand_keys makes each child
+// in registers for its key (and probes a table of remaining depth 2 if one
+// exists), the collapse runs on the keys, the live children are listed by
+// state, and the fused leaf makes each listed child from its parent and move,
+// generates its moves in LDS and counts their moves: perft(2) per child.
+// Collapsed children cost no move count, and the smaller level needs fewer
+// splits, so more children are collapsed.
+int Engine::tt_last(int v, TTable &tt, Pos *B, Key128 *K, u64 n, u32 *cnt, u64 *P, int ply) {
+    const int r = 3, rcd = 2;
+    if (n > 0xFFFFFFFFULL) return tt_split(v, tt, B, K, n, cnt, r, P, ply);
+    const size_t mark = top;
+    u64 *off = (u64 *)alloc((n + 1) * sizeof(u64));
+    if (!off) { top = mark; return tt_split(v, tt, B, K, n, cnt, r, P, ply); }
+    u64 m = 0;
+    int rc = scan(cnt, n, off, &m);
+    if (rc == PG_ENOMEM) { top = mark; return tt_split(v, tt, B, K, n, cnt, r, P, ply); }
+    if (rc) return rc;
+    if (m > 0xFFFFFFFFULL) { top = mark; return tt_split(v, tt, B, K, n, cnt, r, P, ply); }
+    u64 *CP = nullptr;
+    if (m) {
+        const u64 chunks = (m + LIVE_CHUNK - 1) / LIVE_CHUNK;
+        Key128 *CK = (Key128 *)alloc(m * sizeof(Key128));
+        CP = CK ? (u64 *)alloc(m * sizeof(u64)) : nullptr;
+        u32 *D = CP ? (u32 *)alloc(m * sizeof(u32)) : nullptr;
+        u8 *S = D ? (u8 *)alloc(m) : nullptr;
+        u16 *mv = S ? (u16 *)alloc(m * sizeof(u16)) : nullptr;
+        u32 *par = mv ? (u32 *)alloc(m * sizeof(u32)) : nullptr;
+        u32 *chunkCount = par ? (u32 *)alloc(chunks * sizeof(u32)) : nullptr;
+        u64 *chunkOff = chunkCount ? (u64 *)alloc((chunks + 1) * sizeof(u64)) : nullptr;
+        u32 *idx = chunkOff ? (u32 *)alloc(m * sizeof(u32) + 4) : nullptr;
+        const size_t keep = top;
+        const u64 setSlots = dedup ? pow2_at_least(2 * m) : 0;
+        u32 *set = (idx && dedup) ? (u32 *)alloc(setSlots * sizeof(u32)) : nullptr;
+        if (!idx || (dedup && !set)) {
+            top = mark;
+            return tt_split(v, tt, B, K, n, cnt, r, P, ply);
+        }
+        if (ply + 1 < 16) stats.nodes[ply + 1] += m;
+        PG_VARIANT_CALL(v, launch_gen, *this, B, n, off, mv, par);
+        PG_VARIANT_CALL(v, launch_expand_keys, *this, tt, B, K, par, mv, m, CK, CP, S, rcd);
+        if (set) {
+            HIP_TRY(hipMemsetAsync(set, 0, setSlots * sizeof(u32), stream));
+            k_dedup<<<grid(m, 256), 256, 0, stream>>>(nullptr, CK, m, nullptr, S, D, set, setSlots - 1, dErr, tt.dCtr,
+                                                       rcd);
+        }
+        HIP_TRY(hipGetLastError());
+        top = keep;  // the set is no longer needed
+        k_live_count<<<(u32)chunks, LIVE_WG, 0, stream>>>(nullptr, S, m, chunkCount);
+        HIP_TRY(hipGetLastError());
+        u64 nLive = 0;
+        rc = scan(chunkCount, chunks, chunkOff, &nLive);
+        if (rc) return rc;
+        k_live_write<<<(u32)chunks, LIVE_WG, 0, stream>>>(nullptr, S, m, chunkOff, idx);
+        HIP_TRY(hipGetLastError());
+        if (timing) HIP_TRY(hipEventRecord(ev0, stream));
+        if (nLive) PG_VARIANT_CALL(v, launch_leaf_moves_per, *this, B, par, mv, idx, nLive, CP);
+        HIP_TRY(hipGetLastError());
+        if (timing) {
+            HIP_TRY(hipEventRecord(ev1, stream));
+            HIP_TRY(hipEventSynchronize(ev1));
+            float ms = 0;
+            HIP_TRY(hipEventElapsedTime(&ms, ev0, ev1));
+            stats.leaf_kernel_ms += ms;
+        }
+        stats.leaf_launches++;
+        stats.leaf_parents += nLive;
         k_resolve_store<<<grid(m, 256), 256, 0, stream>>>(CK, m, CP, S, D, tt.view, rcd, tt.dCtr);
     }
     k_segsum<<<grid(n, 256), 256, 0, stream>>>(B, n, off, CP, P, ply == 0 ? bias : 0, dOvf);
