@@ -539,12 +539,11 @@ def divide(dev, pos, depth, slider=SLIDER_AUTO, tt=None):
 
 # Tree arena of a context that runs table searches.  A deep table search
 # expands levels of billions of boards (start-position perft(10): 2.6e9 at
-# ply 8, about 85 B each with key, perft, count, state, owner, move and
-# parent); a level that does not fit is split, and equal boards in different
-# parts are not collapsed, so a larger arena means less leaf work: start
-# perft(10) with a 16 GB table takes 1.05 s on 48 GiB (14 splits) and 0.97 s
-# on 144 GiB (2), perft(11) 13.0 s on 48 GiB and 12.1 s on 160 GiB; beyond
-# that the larger collapse set costs what the leaf saves.
+# ply 8, about 43 B each with key, perft, owner, state, move, parent and
+# collapse-set slots); a level that does not fit is split, and equal boards
+# in different parts are not collapsed, so a larger arena means less leaf
+# work: start perft(10) with a 16 GB table takes 0.95 s on 48 GiB (6 splits)
+# and 0.84 s on 160 GiB (none; its deepest level peaks at 159 GB).
 DEFAULT_ARENA = 48 << 30     # the library's default (pg_config.arena_bytes = 0)
 TABLE_ARENA_MAX = 160 << 30
 TABLE_ARENA_RESERVE = 16 << 30
