@@ -430,7 +430,7 @@ static __global__ __launch_bounds__(256) void kd_resolve(Plan *P, int l, TTView 
         const u8 st = C.state[i];
         bool stored = false;
         if (st == ST_DUP) C.perft[i] = C.perft[C.dup[i]];
-        else if (st == ST_LIVE && C.k && rc >= 2) stored = tt_store(tt, rc, C.k[i], C.perft[i]);
+        else if (st == ST_LIVE && C.k && rc >= 2 && tt.base[rc]) stored = tt_store(tt, rc, C.k[i], C.perft[i]);
         if (ctr) ctr_count(ctr, rc, 2, stored);
     }
 }

@@ -230,7 +230,7 @@ static __global__ __launch_bounds__(256) void k_resolve_store(const Key128 *__re
         const u8 st = state[i];
         bool stored = false;
         if (st == ST_DUP) cperft[i] = cperft[dup[i]];
-        else if (st == ST_LIVE && rc >= 2) stored = tt_store(tt, rc, ckeys[i], cperft[i]);
+        else if (st == ST_LIVE && rc >= 2 && tt.base[rc]) stored = tt_store(tt, rc, ckeys[i], cperft[i]);  // no loads without a table
         if (ctr) ctr_count(ctr, rc, 2, stored);
     }
 }
