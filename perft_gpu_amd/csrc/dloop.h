@@ -27,11 +27,12 @@ struct DLevel {
     u32 *dup;
     u64 *tiles;       // scan tile sums
     u32 *live;        // leaf level: the boards with moves (kd_live), when there was room
-    u16 *mv;          // transient: moves / parents of the children
-    u32 *par;
+    u16 *mv;          // moves / parents of the children: transient, or kept (keep) when
+    u32 *par;         //   the children's boards are not stored (the leaf makes them)
     u32 *set;         // transient: duplicate set
     u64 setMask;
     u64 release;      // bump top to return to after expansion
+    u32 keep;         // mv / par outlive the expansion
 };
 
 struct Plan {
@@ -109,8 +110,10 @@ static __global__ __launch_bounds__(256) void kd_root_store(Plan *P, TTView tt, 
 __device__ __forceinline__ void release_level(Plan *P, int l) {
     DLevel &L = P->lv[l];
     if (P->lv[l + 1].n) P->top = L.release;
-    L.mv = nullptr;
-    L.par = nullptr;
+    if (!L.keep) {
+        L.mv = nullptr;
+        L.par = nullptr;
+    }
     L.set = nullptr;
 }
 
@@ -187,14 +190,40 @@ static __global__ __launch_bounds__(SCAN_WG) void kd_scan_down(Plan *P, int l) {
 
 // allocate level l+1 (children of l) and the transient arrays; a total-only
 // search keeps just boards and counts per level, and for its last expansion
-// (total == 2) only the moves and parents: the leaf makes those children
+// (total == 2) only the moves and parents: the leaf makes those children.
+// total == 3: the last level of a search with per-board results and keys
+// keeps no boards or counts, only keys, perfts, states and owners; the moves
+// and parents stay until the leaf, which makes the children from them.
 __device__ void plan_children(Plan *P, int l, int wantKeys, int wantSet, int total) {
     DLevel &L = P->lv[l];
     DLevel &C = P->lv[l + 1];
     memset(&C, 0, sizeof(C));
+    L.keep = 0;
     const u64 m = L.n ? L.off[L.n] : 0;
     if (m == 0 || m > 0xFFFFFFFFULL) {
         if (m) P->overflow = 1;
+        return;
+    }
+    if (total == 3) {
+        C.perft = (u64 *)plan_alloc(P, m * 8);
+        C.state = (u8 *)plan_alloc(P, m);
+        C.dup = (u32 *)plan_alloc(P, m * 4);
+        C.k = (Key128 *)plan_alloc(P, m * sizeof(Key128));
+        L.mv = (u16 *)plan_alloc(P, m * 2);
+        L.par = (u32 *)plan_alloc(P, m * 4);
+        L.keep = 1;
+        L.release = P->top;  // only the set goes after the expansion
+        if (wantSet) {
+            u64 s = 1;
+            while (s < 2 * m) s <<= 1;
+            L.set = (u32 *)plan_alloc(P, s * 4);
+            L.setMask = s - 1;
+        }
+        if (P->overflow) {
+            memset(&C, 0, sizeof(C));
+            return;
+        }
+        C.n = m;
         return;
     }
     if (total == 2) {
@@ -281,6 +310,42 @@ __global__ __launch_bounds__(wg_size<V>()) void kd_expand(Plan *P, int l, TTView
     }
 }
 
+// the last level when it keeps keys only (plan_children, total == 3): each
+// child is made in registers for its key and its table probe
+template <int V>
+__global__ __launch_bounds__(wg_size<V>()) void kd_expand_keys(Plan *P, int l, TTView tt, int rc, TTCounters *ctr,
+                                                      Tables t) {
+    stage_tables<V>(t);
+    const DLevel L = P->lv[l];
+    const DLevel C = P->lv[l + 1];
+    PG_GRID_LOOP(i, C.n) {
+        const u32 pi = L.par[i];
+        Pos p = L.b[pi];
+        Key128 h = L.k[pi];
+        make_any_hash(p, L.mv[i], h, t);
+        u64 v = 0;
+        const bool probe = rc >= 2 && tt.base[rc];
+        const bool hit = probe && tt_probe(tt, rc, h, &v);
+        if (ctr) {
+            ctr_count(ctr, rc, 0, probe);
+            ctr_count(ctr, rc, 1, hit);
+        }
+        C.k[i] = h;
+        C.state[i] = hit ? ST_HIT : ST_LIVE;
+        C.perft[i] = v;
+        C.dup[i] = (u32)i;
+    }
+}
+
+// equal children of level l+1 when their boards are not stored: both made
+// again from parent and move
+__device__ __forceinline__ bool same_child(const DLevel &L, u64 a, u64 b, const Tables &t) {
+    Pos pa = L.b[L.par[a]], pb = L.b[L.par[b]];
+    make_any(pa, L.mv[a], t);
+    make_any(pb, L.mv[b], t);
+    return same_board(pa, pb);
+}
+
 // fill kernel for the duplicate set (graphs get no memset nodes)
 static __global__ __launch_bounds__(256) void kd_clear_set(Plan *P, int l) {
     const DLevel L = P->lv[l];
@@ -291,13 +356,14 @@ static __global__ __launch_bounds__(256) void kd_clear_set(Plan *P, int l) {
 // one child after its first probe (kd_dedup): follow the probe sequence
 // until the child owns a slot or finds its equal
 __device__ __forceinline__ void kd_dedup_finish(Plan *P, const DLevel &L, const DLevel &C, u64 i, bool live, u64 s,
-                                                u32 cur, TTCounters *ctr, int rc, int keyOnly) {
+                                                u32 cur, TTCounters *ctr, int rc, int keyOnly, const Tables &t) {
     bool isDup = false;
     if (live) {
         for (u64 step = 0;; step++) {
             if (cur == 0) break;
             const u32 o = cur - 1;
-            if (C.k[o].lo == C.k[i].lo && C.k[o].hi == C.k[i].hi && (keyOnly || same_board(C.b[o], C.b[i]))) {
+            if (C.k[o].lo == C.k[i].lo && C.k[o].hi == C.k[i].hi &&
+                (keyOnly || (C.b ? same_board(C.b[o], C.b[i]) : same_child(L, o, i, t)))) {
                 C.dup[i] = o;
                 isDup = true;
                 break;
@@ -310,7 +376,7 @@ __device__ __forceinline__ void kd_dedup_finish(Plan *P, const DLevel &L, const 
     if (ctr) ctr_count(ctr, rc, 3, isDup);
     if (isDup) {
         C.state[i] = ST_DUP;
-        C.cnt[i] = 0;
+        if (C.cnt) C.cnt[i] = 0;  // a level that keeps keys only has no counts
     }
 }
 
@@ -318,7 +384,7 @@ __device__ __forceinline__ void kd_dedup_finish(Plan *P, const DLevel &L, const 
 // both first probes in flight).  keyOnly (a search with a table, which
 // accepts hits on the 128-bit key): equal keys are equal positions; without
 // a table the boards are compared as well.
-static __global__ __launch_bounds__(256) void kd_dedup(Plan *P, int l, TTCounters *ctr, int rc, int keyOnly) {
+static __global__ __launch_bounds__(256) void kd_dedup(Plan *P, int l, TTCounters *ctr, int rc, int keyOnly, Tables t) {
     const DLevel L = P->lv[l];
     const DLevel C = P->lv[l + 1];
     if (!L.set) return;
@@ -330,8 +396,8 @@ static __global__ __launch_bounds__(256) void kd_dedup(Plan *P, int l, TTCounter
         const u64 s0 = (C.k[i0].lo >> 7) & L.setMask, s1 = l1 ? (C.k[i1].lo >> 7) & L.setMask : 0;
         const u32 c0 = l0 ? atomicCAS(&L.set[s0], 0u, (u32)i0 + 1) : 0;
         const u32 c1 = l1 ? atomicCAS(&L.set[s1], 0u, (u32)i1 + 1) : 0;
-        kd_dedup_finish(P, L, C, i0, l0, s0, c0, ctr, rc, keyOnly);
-        if (has1) kd_dedup_finish(P, L, C, i1, l1, s1, c1, ctr, rc, keyOnly);
+        kd_dedup_finish(P, L, C, i0, l0, s0, c0, ctr, rc, keyOnly, t);
+        if (has1) kd_dedup_finish(P, L, C, i1, l1, s1, c1, ctr, rc, keyOnly, t);
     }
 }
 
@@ -364,6 +430,12 @@ static __global__ __launch_bounds__(LIVE_WG) void kd_live(Plan *P, int l) {
 #pragma unroll
         for (int k = 0; k < LIVE_PER_THREAD; k++) {
             const u64 i = first + k;
+            if (!L.cnt) {  // a level that keeps keys only: its live children, counted by the leaf
+                const bool lv = i < L.n && L.state[i] == ST_LIVE;
+                if (lv) f |= 1u << k;
+                c += lv;
+                continue;
+            }
             const u32 n = i < L.n ? L.cnt[i] : 0;
             if (n) f |= 1u << k;
             c += n != 0;
@@ -388,6 +460,20 @@ __global__ __launch_bounds__(LEAF_WG) __attribute__((amdgpu_waves_per_eu(V == SL
     const bool listed = !TOTAL && L.live;
     leaf_tiles<V, !TOTAL>(L.b, L.cnt, listed ? P->liveN : L.n, &P->acc, &P->leafItems, L.perft, L.state,
                           l == 0 ? P->bias : 0, &P->err, &P->ovf, t, listed ? L.live : nullptr);
+}
+
+// perft(2) of the children of level l-1 that form level l when level l keeps
+// keys only: the leaf makes each listed child from its parent and move (or,
+// without room for the list, every child, writing the live ones)
+template <int V>
+__global__ __launch_bounds__(LEAF_WG) __attribute__((amdgpu_waves_per_eu(V == SL_KOGGE ? 4 : 5, 8))) void kd_leaf_keys(
+    Plan *P, int l, Tables t) {
+    const DLevel L = P->lv[l];
+    const DLevel A = P->lv[l - 1];
+    const bool listed = L.live != nullptr;
+    leaf_tiles<V, true, true>(A.b, nullptr, listed ? P->liveN : L.n, &P->acc, &P->leafItems, L.perft,
+                              listed ? nullptr : L.state, 0, &P->err, &P->ovf, t, listed ? L.live : nullptr, A.par,
+                              A.mv);
 }
 
 // perft(3) of the boards of level l summed into the plan's total: the

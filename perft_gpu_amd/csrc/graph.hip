@@ -103,6 +103,13 @@ template <int V> static void cap_leaf(hipStream_t s, Plan *P, int l, int g, cons
 template <int V> static void cap_leaf_total(hipStream_t s, Plan *P, int l, int g, const Tables &t) {
     if constexpr (V != SL_BYTE) kd_leaf<V, true><<<g, LEAF_WG, 0, s>>>(P, l, t);
 }
+template <int V> static void cap_expand_keys(hipStream_t s, Plan *P, int l, const TTView &tt, int rc, TTCounters *ctr,
+                                             int g, const Tables &t) {
+    if constexpr (V != SL_BYTE) kd_expand_keys<V><<<g, wg_size<V>(), 0, s>>>(P, l, tt, rc, ctr, t);
+}
+template <int V> static void cap_leaf_keys(hipStream_t s, Plan *P, int l, int g, const Tables &t) {
+    if constexpr (V != SL_BYTE) kd_leaf_keys<V><<<g, LEAF_WG, 0, s>>>(P, l, t);
+}
 template <int V> static void cap_leaf_moves(hipStream_t s, Plan *P, int l, int g, const Tables &t) {
     if constexpr (V != SL_BYTE) kd_leaf_moves<V><<<g, LEAF_WG, 0, s>>>(P, l, t);
 }
@@ -152,29 +159,42 @@ static void enqueue_search(const Engine &e, hipStream_t s, Plan *P, const GraphS
     // levels expanded into children: remaining depth r = D - l >= 3, or >= 2
     // for the byte-lookup variant (no fused leaf, LDS is taken by its tables)
     const int lastExpand = v == SL_BYTE ? D - 2 : D - 3;
+    // with keys, the deepest level keeps keys only (plan_children, total 3):
+    // its children are made for their keys and collapsed, and the leaf makes
+    // those that remain from their parent and move
+    const bool keysLast = sh.keys && v != SL_BYTE && lastExpand >= 0;
     for (int l = 0; l <= lastExpand; l++) {
         const int rc = D - l - 1;
         const int wantSet = sh.dedup && rc >= 2;
+        const bool kl = keysLast && l == lastExpand;
         kd_plan_scan<<<1, 64, 0, s>>>(P, l);
         kd_scan_reduce<<<gScan, SCAN_WG, 0, s>>>(P, l);
-        kd_scan_spine<<<1, 1024, 0, s>>>(P, l, sh.keys, wantSet, 0);
+        kd_scan_spine<<<1, 1024, 0, s>>>(P, l, sh.keys, wantSet, kl ? 3 : 0);
         kd_scan_down<<<gScan, SCAN_WG, 0, s>>>(P, l);
         PG_VARIANT_CALL(v, cap_generate, s, P, l, gv, e.dt);
-        PG_VARIANT_CALL(v, cap_expand, s, P, l, tt, rc, ctr, gv, e.dt);
+        if (kl) {
+            PG_VARIANT_CALL(v, cap_expand_keys, s, P, l, tt, rc, ctr, gv, e.dt);
+        } else {
+            PG_VARIANT_CALL(v, cap_expand, s, P, l, tt, rc, ctr, gv, e.dt);
+        }
         if (wantSet) {
             kd_clear_set<<<gWork, 256, 0, s>>>(P, l);
-            kd_dedup<<<gWork, 256, 0, s>>>(P, l, ctr, rc, sh.tt != nullptr);
+            kd_dedup<<<gWork, 256, 0, s>>>(P, l, ctr, rc, sh.tt != nullptr, e.dt);
         }
     }
     const int last = lastExpand + 1;  // deepest materialized level
     if (D == 1 || v == SL_BYTE) {
         kd_perft1<<<gWork, 256, 0, s>>>(P, last < 0 ? 0 : last);
     } else {
-        // the leaf over the boards with moves (table hits and collapsed
-        // duplicates have none)
+        // the leaf over the live boards (table hits and collapsed duplicates
+        // are resolved already)
         kd_plan_live<<<1, 64, 0, s>>>(P, last);
         kd_live<<<gWork, LIVE_WG, 0, s>>>(P, last);
-        PG_VARIANT_CALL(v, cap_leaf, s, P, last, gLeaf, e.dt);
+        if (keysLast) {
+            PG_VARIANT_CALL(v, cap_leaf_keys, s, P, last, gLeaf, e.dt);
+        } else {
+            PG_VARIANT_CALL(v, cap_leaf, s, P, last, gLeaf, e.dt);
+        }
     }
     for (int l = lastExpand; l >= 0; l--) {
         kd_resolve<<<gWork, 256, 0, s>>>(P, l, tt, D - l - 1, ctr);

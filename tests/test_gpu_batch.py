@@ -94,8 +94,10 @@ def test_batch_equals_single_position_path(dev, pg):
 
 
 def test_batch_splits_when_region_small(pg):
+    # 128 roots at depth 4: about 7 MB for the deepest level (keys, perfts,
+    # owners, states, moves, parents and the collapse set); 2 MiB forces splits
     arr, items = random_set(128)
-    with pg.Device(0, arena_bytes=8 << 20) as small:
+    with pg.Device(0, arena_bytes=2 << 20) as small:
         got, st = small.perft_batch(arr, 4, 0, True)
         assert [int(x) for x in got] == [it["perft"][4] for it in items]
         assert st["splits"] > 0
