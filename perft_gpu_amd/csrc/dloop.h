@@ -332,8 +332,7 @@ __global__ __launch_bounds__(wg_size<V>()) void kd_expand_keys(Plan *P, int l, T
         }
         C.k[i] = h;
         C.state[i] = hit ? ST_HIT : ST_LIVE;
-        C.perft[i] = v;
-        C.dup[i] = (u32)i;
+        if (hit) C.perft[i] = v;  // the leaf writes the live children's, the resolve the duplicates'
     }
 }
 

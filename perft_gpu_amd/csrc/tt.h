@@ -154,7 +154,7 @@ __global__ __launch_bounds__(wg_size<V>()) void k_expand_keys(const Pos *__restr
             ctr_count(ctr, rc, 1, hit);
         }
         ckeys[i] = h;
-        cperft[i] = v;
+        if (hit) cperft[i] = v;  // the leaf writes the live children's, the resolve the duplicates'
         state[i] = hit ? ST_HIT : ST_LIVE;
     }
 }
